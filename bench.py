@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the replica-parallel Metropolis engine on one MI355X.
+
+Workload (multispin throughput): L = 64, periodic, J = 1, beta = 0.54,
+h = 0.07, 16384 replicas (256 words of 64 replicas, gpu_method 2), all spins
+-1 at start, Philox4x32-10 streams.  One step = one 100-sweep launch of the
+sweep kernel with the fused magnetisation / nucleation check, the host reading
+the nucleated count (the driver's mag_output_int cadence); every 10th step
+also dumps the grids (grid_output_int = 1000) as int32 to host memory.  The
+default 500 steps are the full 50000-sweep run; the nucleated fraction curve
+is fitted to 1 - exp(-k (t - t0)).
+
+Prints one JSON line.  `value` is spin-flip attempts per second over the timed
+steps (wall clock, HIP-synchronised on both sides).  `roofline` is for the
+sweep kernel, timed with HIP events on the engine's stream.  `cpu_baseline`
+times the oracle's restatement of the reference CPU sweep (src/mc_cpu.c, per
+replica MT19937 streams) on the host cores over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+L, R, BETA, H = 64, 16384, 0.54, 0.07
+SWEEPS_PER_STEP, DUMP_EVERY_STEPS = 100, 10
+UP_THR, DN_THR = 0.90, -0.90
+# Roofline of the LDS-resident multispin kernel: 5 reads of the 4-byte LDS
+# bank word holding the spin bit per attempt; peak = 256 CU x 128 B/clk x 2.4 GHz.
+LDS_BYTES_PER_ATTEMPT = 20
+LDS_PEAK_GBS = 256 * 128 * 2.4
+METRIC = "spin-flip attempts/sec (1 GPU, L=64) + committor pB estimates/sec at MAE≤0.05"
+
+
+def fit_rate(sweeps: np.ndarray, frac: np.ndarray):
+    """Least-squares fit of 1 - exp(-k (t - t0)) (the reference notebooks' model)."""
+    try:
+        from scipy.optimize import curve_fit
+
+        m = frac > 0
+        if m.sum() < 3:
+            return None, None
+        f = lambda t, k, t0: 1.0 - np.exp(-k * (t - t0))  # noqa: E731
+        (k, t0), _ = curve_fit(f, sweeps[m], frac[m], p0=(1.5e-4, 500.0), maxfev=20000)
+        return float(k), float(t0)
+    except Exception:
+        return None, None
+
+
+def cpu_baseline(max_seconds: float = 12.0) -> dict:
+    """The oracle's mc_cpu.c restatement, all allowed host cores, bounded sample."""
+    from oracle import oracle as O
+
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    nrep = 4 * ncores
+    g = -np.ones((nrep, L, L), dtype=np.int32)
+    # calibrate with one sweep, then size the sample to ~max_seconds
+    t0 = time.perf_counter()
+    O.sweep_mt_per_replica(L, g, 2894203475, 1, BETA, H, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    nsweeps = int(max(1, min(2000, max_seconds / max(dt, 1e-6))))
+    g = -np.ones((nrep, L, L), dtype=np.int32)
+    t0 = time.perf_counter()
+    O.sweep_mt_per_replica(L, g, 2894203475, nsweeps, BETA, H, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    attempts = nrep * nsweeps * L * L
+    return {"value": attempts / dt, "unit": "attempts/s", "cores": ncores, "kind": "port",
+            "sample": f"L={L}, {nrep} replicas x {nsweeps} sweeps from all -1, beta={BETA}, h={H}, "
+                      f"per-replica MT19937 (src/mc_cpu.c arithmetic), {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--method", type=int, default=2)
+    ap.add_argument("--slots", type=int, default=0, help="slots per replica (0 = tuner)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-dump", action="store_true", help="skip the grid dumps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas are independent "
+              "chains and this build measures one GPU's share.", file=sys.stderr)
+        return 2
+
+    import gpu_2dising_amd as G
+
+    G.init_device(-1)
+    G.set_tuning(args.slots, 0)
+    rng = G.RngState(R, 20251202)
+    eng = G.Engine(L, R, args.method, rng, BETA, H)
+    g0 = -np.ones((R, L, L), dtype=np.int32)
+    host = np.empty((R, L * L), dtype=np.int32)
+    S, _ = G.get_tuning(args.method, L, R)
+
+    # warmup (untimed), then restart from all -1
+    eng.set_grids(g0)
+    for _ in range(args.warmup):
+        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
+        eng.counts()
+    eng.set_grids(g0)
+    eng.sync()
+
+    ms0 = eng.kernel_ms()
+    frac = []
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
+        c = eng.counts()  # synchronises
+        frac.append(c[0] / R)
+        if not args.no_dump and (step + 1) % DUMP_EVERY_STEPS == 0:
+            G.check(G.lib().gasp_engine_get_grids(eng.ptr, host.ctypes.data, 0))
+    eng.sync()
+    wall = time.perf_counter() - t0
+    kern_ms = eng.kernel_ms() - ms0
+    eng.close()
+    rng.free()
+
+    attempts = float(args.steps) * SWEEPS_PER_STEP * L * L * R
+    value = attempts / wall
+    kern_rate = attempts / (kern_ms / 1e3)
+    per_launch_ms = kern_ms / max(args.steps, 1)
+    achieved = SWEEPS_PER_STEP * L * L * R * LDS_BYTES_PER_ATTEMPT / (per_launch_ms / 1e3) / 1e9
+    sweeps = np.arange(1, args.steps + 1) * SWEEPS_PER_STEP
+    k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (all spins -1 at start)",
+        "config": {
+            "workload": "multispin throughput: L=64, 16384 replicas (64 per 64-bit word), "
+                        "beta=0.54, h=0.07, all -1, 100 sweeps per step with fused magnetisation "
+                        "and nucleation check, int32 grid dump every 1000 sweeps",
+            "L": L, "replicas": R, "sweeps_per_step": SWEEPS_PER_STEP,
+            "total_sweeps": args.steps * SWEEPS_PER_STEP, "gpu_method": args.method,
+            "rng": "philox4x32-10 per replica", "slots_per_replica": S,
+            "grid_dumps": not args.no_dump,
+        },
+        "roofline": {
+            "bound": "lds",
+            "achieved": achieved,
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / LDS_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_attempt": LDS_BYTES_PER_ATTEMPT,
+            "kernel": "sweep_kernel<LAT_MULTISPIN, RNG_PHILOX, S>",
+            "kernel_ms_per_launch": per_launch_ms,
+            "kernel_attempts_per_s": kern_rate,
+        },
+        "nucleation_fit": {"k_per_sweep": k, "t0": t0fit, "final_fraction": frac[-1] if frac else None,
+                           "reference_k_per_sweep": 1.5032e-4},
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_sweep_multispin_L64.json")
+    if os.path.exists(traffic_file):
+        try:
+            with open(traffic_file) as f:
+                out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+        except Exception:
+            pass
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

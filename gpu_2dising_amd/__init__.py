@@ -1,0 +1,326 @@
+"""MI355X-native replica-parallel Metropolis engine for the 2D Ising model.
+
+Drop-in for GASP's GPU engine (gpu_method 0/1/2).  The engine is the C-ABI
+library ``libgasp_amd.so`` (declared in ``include/gasp_amd.h``) with
+hand-written gfx950 HIP kernels; this package holds its sources (``csrc/``),
+its build script (``build.py``), the ``gasp`` command line program
+(``bin/gasp``), the PyGASP-compatible extension module ``gasp`` and this
+ctypes binding of the C ABI.
+
+Nothing here falls back to a CPU path: if the library is missing, ``lib()``
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libgasp_amd.so")
+CLI_PATH = os.path.join(PKG_DIR, "bin", "gasp")
+
+# Every symbol include/gasp_amd.h declares.
+EXPORTS = (
+    "gpuInitDevice", "gpuInitGrid", "gpuInitRand", "preComputeProbs_gpu", "select_gpu_method",
+    "mc_driver_gpu", "init_grids_uniform", "init_grids_from_file", "init_grids_from_array",
+    "init_fates", "read_input_grid", "write_ising_grids", "bootstrap_errbar",
+    "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
+    "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats",
+    "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
+    "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
+    "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
+    "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
+)
+
+CHECK_NONE, CHECK_COUNT, CHECK_FATES = 0, 1, 2
+
+
+class GaspError(RuntimeError):
+    pass
+
+
+# ------------------------------------------------------------------ C types
+class McGrids(C.Structure):  # mc_grids_t
+    _fields_ = [("L", C.c_int), ("ngrids", C.c_int), ("ising_grids", C.POINTER(C.c_int))]
+
+
+class McSampler(C.Structure):  # mc_sampler_t
+    _fields_ = [("tot_nsweeps", C.c_int), ("mag_output_int", C.c_int),
+                ("grid_output_int", C.c_int)]
+
+
+class McFunction(C.Structure):  # mc_function_t
+    _fields_ = [("itask", C.c_int), ("dn_thr", C.c_double), ("up_thr", C.c_double),
+                ("ninputs", C.c_int), ("initial_spin", C.c_int), ("cv", C.c_char_p),
+                ("result", C.POINTER(C.c_float)), ("filename", C.c_char_p)]
+
+
+class McGpuGrids(C.Structure):  # mc_gpu_grids_t
+    _fields_ = [("L", C.c_int), ("ngrids", C.c_int), ("ising_grids", C.POINTER(C.c_int)),
+                ("d_ising_grids", C.c_void_p), ("d_neighbour_list", C.c_void_p)]
+
+
+class GpuRun(C.Structure):  # gpu_run_t
+    _fields_ = [("d_state", C.c_void_p), ("threadsPerBlock", C.c_int), ("gpu_method", C.c_int)]
+
+
+class RunStats(C.Structure):  # gasp_run_stats_t
+    _fields_ = [("sweep_kernel_ms", C.c_double), ("attempts_per_replica", C.c_double),
+                ("launches", C.c_int), ("slots_per_replica", C.c_int),
+                ("waves_per_group", C.c_int), ("method", C.c_int)]
+
+
+GridOutputFunc = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int,
+                             C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_char_p,
+                             C.c_double, C.c_double, C.c_char_p)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The engine library with argument types set.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GaspError(f"{LIB_PATH} is missing: build it with `python -m gpu_2dising_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)
+    L.gpuInitDevice.argtypes = [C.c_int, ip]
+    L.gpuInitDevice.restype = C.c_int
+    L.gpuInitGrid.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.POINTER(vp), C.POINTER(vp)]
+    L.gpuInitRand.argtypes = [C.c_int, C.c_int, C.c_ulong, C.POINTER(vp)]
+    L.preComputeProbs_gpu.argtypes = [C.c_double, C.c_double]
+    L.select_gpu_method.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.select_gpu_method.restype = C.c_int
+    L.mc_driver_gpu.argtypes = [McGpuGrids, C.c_double, C.c_double, ip, McSampler, McFunction,
+                                GpuRun, GridOutputFunc]
+    L.init_grids_uniform.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.init_grids_uniform.restype = ip
+    L.init_grids_from_file.argtypes = [C.c_int, C.c_int]
+    L.init_grids_from_file.restype = ip
+    L.init_grids_from_array.argtypes = [C.c_int, C.c_int, C.c_int, ip]
+    L.init_grids_from_array.restype = ip
+    L.init_fates.argtypes = [C.c_int]
+    L.init_fates.restype = ip
+    L.read_input_grid.argtypes = [C.c_int, C.c_int, ip]
+    L.write_ising_grids.argtypes = [C.c_int, C.c_int, ip, C.c_int, fp, fp, C.c_char_p,
+                                    C.c_double, C.c_double, C.c_char_p]
+    L.write_ising_grids.restype = C.c_int
+    L.bootstrap_errbar.argtypes = [C.c_int, C.c_int, ip]
+    L.bootstrap_errbar.restype = C.c_double
+    L.gasp_last_error.restype = C.c_int
+    L.gasp_error_string.restype = C.c_char_p
+    L.gasp_init_rand_mt.argtypes = [C.c_int, C.c_ulong, C.POINTER(vp)]
+    L.gasp_init_rand_mt.restype = C.c_int
+    L.gasp_free.argtypes = [vp]
+    L.gasp_set_tuning.argtypes = [C.c_int, C.c_int]
+    L.gasp_get_tuning.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip]
+    L.gasp_get_run_stats.argtypes = [C.POINTER(RunStats)]
+    L.gasp_engine_create.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double,
+                                     C.POINTER(vp)]
+    for name in ("gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids",
+                 "gasp_engine_get_grids", "gasp_engine_sweep", "gasp_engine_sync",
+                 "gasp_engine_get_magnetisation", "gasp_engine_get_counts",
+                 "gasp_engine_set_fates", "gasp_engine_get_fates",
+                 "gasp_engine_largest_cluster"):
+        getattr(L, name).restype = C.c_int
+    L.gasp_engine_destroy.argtypes = [vp]
+    L.gasp_engine_set_grids.argtypes = [vp, vp, C.c_int]
+    L.gasp_engine_get_grids.argtypes = [vp, vp, C.c_int]
+    L.gasp_engine_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.gasp_engine_sync.argtypes = [vp]
+    L.gasp_engine_get_magnetisation.argtypes = [vp, fp]
+    L.gasp_engine_get_counts.argtypes = [vp, ip]
+    L.gasp_engine_set_fates.argtypes = [vp, ip]
+    L.gasp_engine_get_fates.argtypes = [vp, ip]
+    L.gasp_engine_largest_cluster.argtypes = [vp, C.c_int, fp]
+    L.gasp_engine_kernel_ms.argtypes = [vp]
+    L.gasp_engine_kernel_ms.restype = C.c_double
+    L.gasp_engine_stream.argtypes = [vp]
+    L.gasp_engine_stream.restype = vp
+    _lib = L
+    return L
+
+
+def check(rc: int = None) -> None:
+    """Raise GaspError if the library recorded an error."""
+    code = lib().gasp_last_error() if rc is None else rc
+    if code:
+        raise GaspError(f"gasp error {code}: {lib().gasp_error_string().decode()}")
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ------------------------------------------------------------------ helpers
+class RngState:
+    """Owns a gasp_rng_state (Philox by default, MT19937 parity streams if mt=True)."""
+
+    def __init__(self, ngrids: int, seed: int, mt: bool = False):
+        self.ptr = C.c_void_p()
+        if mt:
+            check(lib().gasp_init_rand_mt(ngrids, seed, C.byref(self.ptr)))
+        else:
+            lib().gpuInitRand(ngrids, 64, seed, C.byref(self.ptr))
+            check()
+
+    def free(self):
+        if self.ptr:
+            lib().gasp_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """Stream-level engine over the C ABI (gasp_engine_*)."""
+
+    def __init__(self, L: int, ngrids: int, method: int, rng: RngState, beta: float, h: float):
+        self.L, self.R, self.method = L, ngrids, method
+        self.rng = rng
+        self.ptr = C.c_void_p()
+        check(lib().gasp_engine_create(L, ngrids, method, rng.ptr, beta, h, C.byref(self.ptr)))
+
+    def set_grids(self, grids: np.ndarray) -> None:
+        g = np.ascontiguousarray(grids, dtype=np.int32).reshape(self.R, self.L * self.L)
+        check(lib().gasp_engine_set_grids(self.ptr, g.ctypes.data, 0))
+
+    def get_grids(self) -> np.ndarray:
+        out = np.empty((self.R, self.L, self.L), dtype=np.int32)
+        check(lib().gasp_engine_get_grids(self.ptr, out.ctypes.data, 0))
+        return out
+
+    def sweep(self, nsweeps: int, check_mode: int = CHECK_NONE, dn: float = -0.9,
+              up: float = 0.9) -> None:
+        check(lib().gasp_engine_sweep(self.ptr, nsweeps, check_mode, dn, up))
+
+    def sync(self) -> None:
+        check(lib().gasp_engine_sync(self.ptr))
+
+    def magnetisation(self) -> np.ndarray:
+        out = np.empty(self.R, dtype=np.float32)
+        check(lib().gasp_engine_get_magnetisation(self.ptr, _fp(out)))
+        return out
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.int32)
+        check(lib().gasp_engine_get_counts(self.ptr, _ip(out)))
+        return out
+
+    def set_fates(self, fates: np.ndarray) -> None:
+        f = np.ascontiguousarray(fates, dtype=np.int32)
+        check(lib().gasp_engine_set_fates(self.ptr, _ip(f)))
+
+    def fates(self) -> np.ndarray:
+        out = np.empty(self.R, dtype=np.int32)
+        check(lib().gasp_engine_get_fates(self.ptr, _ip(out)))
+        return out
+
+    def largest_cluster(self, spin: int) -> np.ndarray:
+        out = np.empty(self.R, dtype=np.float32)
+        check(lib().gasp_engine_largest_cluster(self.ptr, spin, _fp(out)))
+        return out
+
+    def kernel_ms(self) -> float:
+        return lib().gasp_engine_kernel_ms(self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            check(lib().gasp_engine_destroy(self.ptr))
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def set_tuning(slots: int = 0, waves: int = 0) -> None:
+    lib().gasp_set_tuning(slots, waves)
+
+
+def get_tuning(method: int, L: int, ngrids: int) -> tuple[int, int]:
+    s, w = C.c_int(), C.c_int()
+    lib().gasp_get_tuning(method, L, ngrids, C.byref(s), C.byref(w))
+    return s.value, w.value
+
+
+def run_stats() -> RunStats:
+    st = RunStats()
+    lib().gasp_get_run_stats(C.byref(st))
+    return st
+
+
+def init_device(index: int = -1) -> int:
+    """gpuInitDevice; returns the CU count.  Raises if no GPU is visible."""
+    n = C.c_int(0)
+    dev = lib().gpuInitDevice(index, C.byref(n))
+    if dev < 0:
+        raise GaspError("no GPU visible to the engine")
+    return n.value
+
+
+def run_driver(L: int, grids: np.ndarray, *, beta: float, h: float, rng: RngState, method: int,
+               tot_nsweeps: int, mag_output_int: int, grid_output_int: int, itask: int = 0,
+               dn_thr: float = -0.9, up_thr: float = 0.9, ninputs: int = 1,
+               cv: str = "magnetisation", initial_spin: int = -1, threads_per_block: int = 64,
+               fates: np.ndarray | None = None, outfunc=None) -> dict:
+    """mc_driver_gpu through the C ABI (gpuInitGrid, preComputeProbs_gpu, mc_driver_gpu)."""
+    Lb = lib()
+    R = grids.shape[0]
+    host = np.ascontiguousarray(grids, dtype=np.int32).reshape(R, L * L).copy()
+    d_grids, d_nb = C.c_void_p(), C.c_void_p()
+    Lb.gpuInitGrid(L, R, threads_per_block, _ip(host), C.byref(d_grids), C.byref(d_nb))
+    check()
+    Lb.preComputeProbs_gpu(beta, h)
+    nres = tot_nsweeps // mag_output_int if itask == 0 else ninputs
+    result = np.zeros(max(nres, 1), dtype=np.float32)
+    if fates is None:
+        fates = np.full(R, -1, dtype=np.int32)
+    fates = np.ascontiguousarray(fates, dtype=np.int32)
+    g = McGpuGrids(L, R, _ip(host), d_grids, d_nb)
+    s = McSampler(tot_nsweeps, mag_output_int, grid_output_int)
+    f = McFunction(itask, dn_thr, up_thr, ninputs, initial_spin, cv.encode(), _fp(result),
+                   b"gridstates.bin")
+    run = GpuRun(rng.ptr, threads_per_block, method)
+    cb = GridOutputFunc(outfunc) if outfunc is not None else GridOutputFunc(0)
+    try:
+        Lb.mc_driver_gpu(g, beta, h, _ip(fates), s, f, run, cb)
+        check()
+        final = np.empty((R, L * L), dtype=np.int32)
+        lib().gasp_clear_error()
+        # final device grids (the driver leaves the last state in d_ising_grids)
+        import ctypes.util  # noqa: F401
+        _d2h(final, d_grids)
+    finally:
+        Lb.gasp_free(d_grids)
+        Lb.gasp_free(d_nb)
+    return {"result": result[:nres], "fates": fates, "host_grids": host.reshape(R, L, L),
+            "final_grids": final.reshape(R, L, L), "stats": run_stats()}
+
+
+_hip = None
+
+
+def _d2h(dst: np.ndarray, dptr: C.c_void_p) -> None:
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    rc = _hip.hipMemcpy(dst.ctypes.data, dptr, dst.nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc:
+        raise GaspError(f"hipMemcpy failed ({rc})")

@@ -1,0 +1,1025 @@
+// gasp_engine.hip -- host side of the MI355X replica-parallel Metropolis engine:
+// device set-up, random streams, the engine object that owns the replicas'
+// lattices in a method's native layout, and mc_driver_gpu.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "gasp_internal.h"
+#include "gasp_kernels.h"
+
+using namespace gasp;
+
+// =================================================================== errors
+namespace {
+thread_local int t_err = GASP_OK;
+thread_local std::string t_msg;
+void (*g_print_cb)(const char *) = nullptr;
+int g_style = 0;
+}  // namespace
+
+namespace gasp {
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = code;
+    t_msg = buf;
+    fprintf(stderr, "%s\n", buf);
+    fflush(stderr);
+    return code;
+}
+void clear_error() {
+    t_err = GASP_OK;
+    t_msg.clear();
+}
+void emit(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (g_print_cb) {
+        g_print_cb(buf);
+    } else {
+        fputs(buf, stdout);
+        fflush(stdout);
+    }
+}
+int output_style() { return g_style; }
+uint64_t threshold_of(double p) {
+    // xi = (u + 0.5) 2^-32 is exact in double, and so is p 2^32 - 0.5 for
+    // p < 1:  xi < p  <=>  u < ceil(p 2^32 - 0.5).
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return 4294967296ull;
+    const double y = ceil(p * 4294967296.0 - 0.5);
+    return y <= 0.0 ? 0ull : (uint64_t)y;
+}
+}  // namespace gasp
+
+extern "C" int gasp_last_error(void) { return t_err; }
+extern "C" const char *gasp_error_string(void) { return t_msg.c_str(); }
+extern "C" void gasp_clear_error(void) { clear_error(); }
+extern "C" void gasp_set_output(void (*cb)(const char *), int style) {
+    g_print_cb = cb;
+    g_style = style;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                         \
+    } while (0)
+
+// ============================================================ process state
+namespace {
+
+// Single-precision acceptance table of preComputeProbs_gpu and its integer
+// image (process-global, as the reference's __constant__ d_Pacc).
+float g_pacc_f32[20];
+uint64_t g_thr_f32[10];
+
+int g_tune_slots = 0;
+int g_tune_waves = 0;
+thread_local gasp_run_stats_t t_stats{};
+
+// Thresholds in kernel order j = 5*s01 + (number of up neighbours).  With
+// s = 2 s01 - 1 and nsum = 2 c - 4 the reference index 5(s+1) + nsum + 4 is
+// 10 s01 + 2 c.
+void thr_from_table_f(const float pacc[20], uint64_t out[10]) {
+    for (int s01 = 0; s01 < 2; ++s01)
+        for (int c = 0; c <= 4; ++c) out[5 * s01 + c] = threshold_of((double)pacc[10 * s01 + 2 * c]);
+}
+void thr_from_table_d(const double pacc[20], uint64_t out[10]) {
+    for (int s01 = 0; s01 < 2; ++s01)
+        for (int c = 0; c <= 4; ++c) out[5 * s01 + c] = threshold_of(pacc[10 * s01 + 2 * c]);
+}
+void table_f32(double beta, double h, float pacc[20]) {
+    // src/mc_gpu.cu:25-31, evaluated on the host in single precision
+    memset(pacc, 0, 20 * sizeof(float));
+    for (int s = -1; s < 2; s += 2)
+        for (int nsum = -4; nsum < 5; nsum += 2) {
+            const int idx = 5 * (s + 1) + nsum + 4;
+            const float de = 2.0f * (float)s * ((float)nsum + (float)h);
+            pacc[idx] = expf(-(float)beta * de);
+        }
+}
+void table_f64(double beta, double h, double pacc[20]) {
+    // src/mc_cpu.c:28-35
+    memset(pacc, 0, 20 * sizeof(double));
+    for (int s = -1; s < 2; s += 2)
+        for (int nsum = -4; nsum < 5; nsum += 2) {
+            const int idx = 5 * (s + 1) + nsum + 4;
+            const double de = 2.0 * (double)s * ((double)nsum + h);
+            pacc[idx] = exp(-beta * de);
+        }
+}
+
+struct DeviceInfo {
+    int ok = 0;
+    int ncu = 0;
+    size_t lds_block = 65536;
+    size_t lds_cu = 65536;
+};
+DeviceInfo device_info() {
+    DeviceInfo d;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return d;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return d;
+    d.ok = 1;
+    d.ncu = p.multiProcessorCount;
+    d.lds_block = p.sharedMemPerBlock;
+    d.lds_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : p.sharedMemPerBlock;
+    return d;
+}
+
+}  // namespace
+
+// ============================================================ random streams
+// gasp_rng_state: a device allocation whose first 64 bytes are this header;
+// per-replica MT19937 states (interleaved [624][ngrids]) follow at offset 256.
+struct RngHeader {
+    uint32_t magic;
+    uint32_t kind;  // 0 = Philox4x32-10, 1 = MT19937 per replica
+    uint64_t seed;
+    int32_t ngrids;
+    int32_t reserved0;
+    uint64_t attempts;  // attempts consumed per replica
+    uint64_t reserved[4];
+};
+static_assert(sizeof(RngHeader) == 64, "header size");
+static constexpr uint32_t kRngMagic = 0x474E5247u;  // "GRNG"
+static constexpr size_t kRngMtOffset = 256;
+
+enum { KIND_PHILOX = 0, KIND_MT = 1 };
+
+static int rng_alloc(int ngrids, uint64_t seed, int kind, gasp_rng_state **out) {
+    if (!out) return set_error(GASP_ERR_ARG, "gpuInitRand: NULL output pointer");
+    *out = nullptr;
+    if (ngrids < 1) return set_error(GASP_ERR_ARG, "gpuInitRand: ngrids must be positive");
+    size_t bytes = kRngMtOffset;
+    if (kind == KIND_MT) bytes += (size_t)624 * 4 * (size_t)ngrids;
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, bytes));
+    RngHeader h{};
+    h.magic = kRngMagic;
+    h.kind = (uint32_t)kind;
+    h.seed = seed;
+    h.ngrids = ngrids;
+    h.attempts = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
+    if (e == hipSuccess && kind == KIND_MT) {
+        uint32_t *st = (uint32_t *)((char *)d + kRngMtOffset);
+        hipLaunchKernelGGL(mt_seed_kernel, dim3((ngrids + 255) / 256), dim3(256), 0, 0, st, ngrids, seed);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(e), __FILE__, __LINE__);
+    }
+    *out = (gasp_rng_state *)d;
+    return GASP_OK;
+}
+
+extern "C" void gpuInitRand(int ngrids, int threadsPerBlock, unsigned long rngseed,
+                            gasp_rng_state **d_state) {
+    (void)threadsPerBlock;
+    clear_error();
+    rng_alloc(ngrids, (uint64_t)rngseed, KIND_PHILOX, d_state);
+}
+
+extern "C" int gasp_init_rand_mt(int ngrids, unsigned long rngseed, gasp_rng_state **d_state) {
+    clear_error();
+    return rng_alloc(ngrids, (uint64_t)rngseed, KIND_MT, d_state);
+}
+
+extern "C" void gasp_free(void *d_ptr) {
+    if (d_ptr) (void)hipFree(d_ptr);
+}
+
+// ============================================================ tuning
+extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
+    g_tune_slots = slots_per_replica;
+    g_tune_waves = waves_per_group;
+}
+
+static int pow2_ceil(long long x) {
+    int p = 1;
+    while (p < x && p < 64) p <<= 1;
+    return p;
+}
+
+// Native layout strides.
+static int layout_stride(int method, int N) {
+    if (method == 2) return N + (N & 1);          // 64-bit words per 64-replica group
+    if (method == 1) return (N + 31) / 32;        // dwords per replica
+    return ((N + 15) / 16) * 16;                  // bytes per replica
+}
+
+static size_t lds_bytes_for(int method, int N, int S, int waves) {
+    const int stride = layout_stride(method, N);
+    if (method == 2) return (size_t)kLdsLattice + 8 * (size_t)stride;
+    if (method == 1) {
+        const int rpw = std::max(1, waves * 64 / S);
+        return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
+    }
+    return (size_t)kLdsLattice;
+}
+
+// Chooses slots per replica S and waves per workgroup.  Aim: about four
+// resident waves per SIMD (16 per CU) over the whole device, within the LDS
+// budget; explicit settings from gasp_set_tuning win when they are valid.
+static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
+    DeviceInfo d = device_info();
+    const int ncu = d.ok ? d.ncu : 256;
+    const size_t lds_block = d.ok ? d.lds_block : 65536;
+    const int N = L * L;
+    int W = (method == 2) ? 1 : (g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1);
+    int Smax = (method == 2) ? 16 : 64;
+    int S;
+    if (g_tune_slots > 0) {
+        S = std::min(pow2_ceil(g_tune_slots), Smax);
+    } else {
+        const long long target_lanes = 16LL * ncu * 64;
+        S = std::min(pow2_ceil((target_lanes + R - 1) / std::max(R, 1)), Smax);
+        if (method == 0) {
+            // each replica's bytes are fetched from L2/HBM: more lanes per
+            // replica hide the latency of large lattices
+            if (N >= 16384) S = std::max(S, 16);
+        }
+    }
+    if (method == 1) {
+        // enough slots so that the replica regions of one workgroup fit
+        while (S < 64 && lds_bytes_for(1, N, S, W) > lds_block) S <<= 1;
+        while (W > 1 && lds_bytes_for(1, N, S, W) > lds_block) W >>= 1;
+    }
+    *S_out = S;
+    *W_out = W;
+}
+
+extern "C" void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
+                                int *waves_per_group) {
+    int S = 1, W = 1;
+    choose_tuning(method, L, ngrids, &S, &W);
+    if (slots_per_replica) *slots_per_replica = S;
+    if (waves_per_group) *waves_per_group = W;
+}
+
+extern "C" void gasp_get_run_stats(gasp_run_stats_t *out) {
+    if (out) *out = t_stats;
+}
+
+// ============================================================ kernel table
+using SweepFn = void (*)(SweepArgs);
+
+template <int LAT, int RNG>
+static SweepFn pick_sweep(int S) {
+    switch (S) {
+        case 1: return sweep_kernel<LAT, RNG, 1>;
+        case 2: return sweep_kernel<LAT, RNG, 2>;
+        case 4: return sweep_kernel<LAT, RNG, 4>;
+        case 8: return sweep_kernel<LAT, RNG, 8>;
+        case 16: return sweep_kernel<LAT, RNG, 16>;
+        default: break;
+    }
+    if constexpr (LAT != LAT_MULTISPIN) {
+        if (S == 32) return sweep_kernel<LAT, RNG, 32>;
+        if (S == 64) return sweep_kernel<LAT, RNG, 64>;
+    }
+    return nullptr;
+}
+
+static SweepFn sweep_fn(int method, int rng, int S) {
+    if (method == 2) return rng == 0 ? pick_sweep<LAT_MULTISPIN, RNG_PHILOX>(S) : pick_sweep<LAT_MULTISPIN, RNG_DRAWS>(S);
+    if (method == 1) return rng == 0 ? pick_sweep<LAT_BITS, RNG_PHILOX>(S) : pick_sweep<LAT_BITS, RNG_DRAWS>(S);
+    return rng == 0 ? pick_sweep<LAT_HBM, RNG_PHILOX>(S) : pick_sweep<LAT_HBM, RNG_DRAWS>(S);
+}
+
+// ============================================================ engine object
+struct gasp_engine {
+    int L = 0, N = 0, R = 0, method = 0;
+    int S = 1, W = 1, stride = 0, rpw = 0, nblocks = 0, block = 0;
+    size_t lds = 0;
+    void *d_lat = nullptr;
+    size_t lat_bytes = 0;
+    float *d_mag = nullptr;
+    float *d_lclus = nullptr;
+    int *d_fate = nullptr;
+    int *d_counts = nullptr;
+    uint64_t *d_thr = nullptr;
+    int8_t *d_spins8 = nullptr;
+    int *d_tmp32 = nullptr;
+    uint32_t *d_draws = nullptr;
+    uint64_t draws_cap = 0;  // attempts per replica that fit in d_draws
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev_pending;  // start/stop pairs not yet accumulated
+    double kernel_ms = 0.0;
+    int launches = 0;
+    gasp_rng_state *d_rng = nullptr;
+    RngHeader rng{};
+    uint64_t thr[10]{};
+};
+
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+static void engine_free(gasp_engine *e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (hipEvent_t ev : e->ev_pending) (void)hipEventDestroy(ev);
+    dfree(e->d_lat);
+    dfree(e->d_mag);
+    dfree(e->d_lclus);
+    dfree(e->d_fate);
+    dfree(e->d_counts);
+    dfree(e->d_thr);
+    dfree(e->d_spins8);
+    dfree(e->d_tmp32);
+    dfree(e->d_draws);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const uint64_t thr[10],
+                      int thr_from_double, double beta, double h, gasp_engine **out) {
+    *out = nullptr;
+    if (L < 2 || L > 8192) return set_error(GASP_ERR_ARG, "Error: L must lie in [2, 8192]");
+    if (R < 1) return set_error(GASP_ERR_ARG, "Error: ngrids must be positive");
+    if (method < 0 || method > 2) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
+    if (!d_state) return set_error(GASP_ERR_ARG, "Error: RNG state is NULL (call gpuInitRand)");
+    gasp_engine *e = new gasp_engine();
+    e->L = L;
+    e->N = L * L;
+    e->R = R;
+    e->method = method;
+    e->d_rng = d_state;
+    hipError_t he = hipMemcpy(&e->rng, d_state, sizeof(RngHeader), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) {
+        engine_free(e);
+        return set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(he), __FILE__, __LINE__);
+    }
+    if (e->rng.magic != kRngMagic) {
+        engine_free(e);
+        return set_error(GASP_ERR_ARG, "Error: RNG state was not made by gpuInitRand");
+    }
+    if (e->rng.ngrids < R) {
+        engine_free(e);
+        return set_error(GASP_ERR_ARG, "Error: RNG state holds %d streams, need %d", e->rng.ngrids, R);
+    }
+    if (e->rng.kind == KIND_MT && e->rng.ngrids != R) {
+        engine_free(e);
+        return set_error(GASP_ERR_ARG, "Error: MT19937 state was made for %d replicas, run has %d",
+                         e->rng.ngrids, R);
+    }
+    if (e->rng.kind == KIND_MT || thr_from_double) {
+        double pd[20];
+        table_f64(beta, h, pd);
+        thr_from_table_d(pd, e->thr);
+    } else {
+        memcpy(e->thr, thr, sizeof e->thr);
+    }
+
+    choose_tuning(method, L, R, &e->S, &e->W);
+    e->stride = layout_stride(method, e->N);
+    DeviceInfo di = device_info();
+    e->lds = lds_bytes_for(method, e->N, e->S, e->W);
+    if (method == 2) {
+        e->block = 64 * e->S;
+        e->nblocks = (R + 63) / 64;
+        e->lat_bytes = (size_t)e->nblocks * e->stride * 8;
+        e->rpw = 64;
+    } else {
+        e->block = 64 * e->W;
+        e->rpw = std::max(1, e->W * 64 / e->S);
+        e->nblocks = (R + e->rpw - 1) / e->rpw;
+        e->lat_bytes = method == 1 ? (size_t)R * e->stride * 4 : (size_t)R * e->stride;
+    }
+    if (di.ok && e->lds > di.lds_block) {
+        const size_t need = e->lds;
+        engine_free(e);
+        return set_error(GASP_ERR_ARG,
+                         "Error: gpu_method %d needs %zu bytes of LDS per workgroup at L=%d (device "
+                         "limit %zu); use select_gpu_method",
+                         method, need, L, (size_t)di.lds_block);
+    }
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S);
+    if (!fn) {
+        engine_free(e);
+        return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
+    }
+#define ETRY(x)                                                                                   \
+    do {                                                                                          \
+        hipError_t q_ = (x);                                                                      \
+        if (q_ != hipSuccess) {                                                                   \
+            engine_free(e);                                                                       \
+            return set_error(q_ == hipErrorOutOfMemory ? GASP_ERR_NOMEM : GASP_ERR_HIP,           \
+                             "GPUassert: %s %s %d", hipGetErrorString(q_), __FILE__, __LINE__);   \
+        }                                                                                         \
+    } while (0)
+    ETRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    ETRY(hipMalloc(&e->d_lat, e->lat_bytes));
+    ETRY(hipMemsetAsync(e->d_lat, 0, e->lat_bytes, e->stream));
+    ETRY(hipMalloc(&e->d_mag, (size_t)R * sizeof(float)));
+    ETRY(hipMalloc(&e->d_fate, (size_t)R * sizeof(int)));
+    ETRY(hipMalloc(&e->d_counts, 4 * sizeof(int)));
+    ETRY(hipMalloc(&e->d_thr, 10 * sizeof(uint64_t)));
+    ETRY(hipMemcpyAsync(e->d_thr, e->thr, sizeof e->thr, hipMemcpyHostToDevice, e->stream));
+    ETRY(hipMemsetAsync(e->d_fate, 0xff, (size_t)R * sizeof(int), e->stream));  // -1
+    ETRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
+    ETRY(hipMemsetAsync(e->d_mag, 0, (size_t)R * sizeof(float), e->stream));
+    if (e->lds > 65536) ETRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds));
+    if (e->rng.kind == KIND_MT) {
+        // draw stream for up to ~512 MiB per batch (12 bytes per attempt per replica)
+        const uint64_t budget = 512ull << 20;
+        e->draws_cap = std::max<uint64_t>(1, budget / (12ull * (uint64_t)R));
+        ETRY(hipMalloc(&e->d_draws, (size_t)e->draws_cap * 12 * (size_t)R));
+    }
+    ETRY(hipStreamSynchronize(e->stream));
+#undef ETRY
+    *out = e;
+    return GASP_OK;
+}
+
+static int engine_collect_events(gasp_engine *e) {
+    for (size_t i = 0; i + 1 < e->ev_pending.size(); i += 2) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev_pending[i], e->ev_pending[i + 1]));
+        e->kernel_ms += ms;
+        (void)hipEventDestroy(e->ev_pending[i]);
+        (void)hipEventDestroy(e->ev_pending[i + 1]);
+    }
+    e->ev_pending.clear();
+    return GASP_OK;
+}
+
+static int engine_sync(gasp_engine *e) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return engine_collect_events(e);
+}
+
+static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *draws, int check,
+                        double dn, double up) {
+    SweepArgs a{};
+    a.lattice = e->d_lat;
+    a.L = e->L;
+    a.N = e->N;
+    a.R = e->R;
+    a.rpw = e->rpw;
+    a.stride = e->stride;
+    a.a_begin = e->rng.attempts;
+    a.n_attempts = n_attempts;
+    a.seed = e->rng.seed;
+    a.draws = draws;
+    a.thr = e->d_thr;
+    a.mag = e->d_mag;
+    a.fate = e->d_fate;
+    a.counts = e->d_counts;
+    a.check = check;
+    a.dn_thr = dn;
+    a.up_thr = up;
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S);
+    if (check != CHECK_NONE) HIP_TRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, e->stream));
+    hipLaunchKernelGGL(fn, dim3(e->nblocks), dim3(e->block), e->lds, e->stream, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev1, e->stream));
+    e->ev_pending.push_back(ev0);
+    e->ev_pending.push_back(ev1);
+    e->launches++;
+    e->rng.attempts += n_attempts;
+    return GASP_OK;
+}
+
+// Advance all replicas by n attempts; `check` applies to the state at the end.
+static int engine_advance(gasp_engine *e, uint64_t n, int check, double dn, double up) {
+    if (e->rng.kind != KIND_MT || n == 0) return launch_sweep(e, n, nullptr, check, dn, up);
+    uint64_t left = n;
+    while (left > 0) {
+        const uint64_t nb = std::min(left, e->draws_cap);
+        uint32_t *st = (uint32_t *)((char *)e->d_rng + kRngMtOffset);
+        hipLaunchKernelGGL(mt_draws_kernel, dim3(e->R), dim3(64), 0, e->stream, st, e->R,
+                           3ull * e->rng.attempts, 3ull * nb, e->d_draws);
+        HIP_TRY(hipGetLastError());
+        const int rc = launch_sweep(e, nb, e->d_draws, left == nb ? check : CHECK_NONE, dn, up);
+        if (rc) return rc;
+        left -= nb;
+    }
+    return GASP_OK;
+}
+
+static int engine_set_grids_impl(gasp_engine *e, const int *grids, int on_device) {
+    const size_t bytes = (size_t)e->R * e->N * sizeof(int);
+    const int *src = grids;
+    if (!on_device) {
+        if (!e->d_tmp32) HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+        HIP_TRY(hipMemcpyAsync(e->d_tmp32, grids, bytes, hipMemcpyHostToDevice, e->stream));
+        src = e->d_tmp32;
+    }
+    const int N = e->N, R = e->R;
+    const dim3 grid(1024), blk(256);
+    if (e->method == 2) {
+        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (uint64_t *)e->d_lat, R, N, e->stride);
+    } else if (e->method == 1) {
+        hipLaunchKernelGGL(pack_bits, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride);
+    } else {
+        hipLaunchKernelGGL(pack_hbm, grid, blk, 0, e->stream, src, (unsigned char *)e->d_lat, R, N, e->stride);
+    }
+    HIP_TRY(hipGetLastError());
+    return GASP_OK;
+}
+
+template <typename T>
+static int engine_unpack(gasp_engine *e, T *dst) {
+    const int N = e->N, R = e->R;
+    const dim3 grid(1024), blk(256);
+    if (e->method == 2) {
+        hipLaunchKernelGGL(unpack_multispin<T>, grid, blk, 0, e->stream, (const uint64_t *)e->d_lat, dst, R, N,
+                           e->stride);
+    } else if (e->method == 1) {
+        hipLaunchKernelGGL(unpack_bits<T>, grid, blk, 0, e->stream, (const unsigned *)e->d_lat, dst, R, N, e->stride);
+    } else {
+        hipLaunchKernelGGL(unpack_hbm<T>, grid, blk, 0, e->stream, (const unsigned char *)e->d_lat, dst, R, N,
+                           e->stride);
+    }
+    HIP_TRY(hipGetLastError());
+    return GASP_OK;
+}
+
+static int engine_get_grids_impl(gasp_engine *e, int *grids, int on_device) {
+    const size_t bytes = (size_t)e->R * e->N * sizeof(int);
+    if (on_device) return engine_unpack<int>(e, grids);
+    if (!e->d_tmp32) HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+    int rc = engine_unpack<int>(e, e->d_tmp32);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(grids, e->d_tmp32, bytes, hipMemcpyDeviceToHost, e->stream));
+    return engine_sync(e);
+}
+
+// ------------------------------------------------------------ largest cluster
+// One workgroup per replica.  Labels and per-root sizes live in LDS when
+// 8*N bytes fit, else in a global workspace.  Lock-free union-find: every
+// site of the target spin starts as its own root; each site unites with its
+// right and lower neighbours (periodic) by hooking the larger root under the
+// smaller with atomicMin and retrying from the returned value when another
+// lane won the race; a final pass points every site at its root, sizes are
+// counted with atomics and the largest is reduced.
+template <bool USE_LDS>
+__global__ void __launch_bounds__(256) cluster_kernel(const int8_t *spins, int N, int L, int spin,
+                                                      int *work, float *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int wmax[4];
+    const int r = blockIdx.x;
+    const int8_t *g = spins + (size_t)r * N;
+    int *lab, *cnt;
+    if constexpr (USE_LDS) {
+        lab = (int *)smem;
+        cnt = lab + N;
+    } else {
+        lab = work + (size_t)r * 2 * N;
+        cnt = lab + N;
+    }
+    const int t = threadIdx.x;
+    for (int i = t; i < N; i += blockDim.x) {
+        lab[i] = (g[i] == spin) ? i : -1;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    auto find = [&](int x) {
+        int y = lab[x];
+        while (y != x) {
+            x = y;
+            y = lab[x];
+        }
+        return x;
+    };
+    for (int i = t; i < N; i += blockDim.x) {
+        if (lab[i] < 0) continue;
+        const int row = i / L, col = i - row * L;
+        const int nb[2] = {row * L + (col == L - 1 ? 0 : col + 1), (row == L - 1 ? 0 : row + 1) * L + col};
+        for (int k = 0; k < 2; ++k) {
+            if (g[nb[k]] != spin) continue;
+            int a = find(i), b = find(nb[k]);
+            while (a != b) {
+                if (a < b) {
+                    const int tmp = a;
+                    a = b;
+                    b = tmp;
+                }
+                // a is the larger root: hook it under b
+                const int old = atomicMin(&lab[a], b);
+                if (old == a) break;
+                a = find(old);
+                b = find(b);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < N; i += blockDim.x)
+        if (lab[i] >= 0) atomicAdd(&cnt[find(i)], 1);
+    __syncthreads();
+    int best = 0;
+    for (int i = t; i < N; i += blockDim.x) best = max(best, cnt[i]);
+    for (int off = 32; off >= 1; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+    if ((t & 63) == 0) wmax[t >> 6] = best;
+    __syncthreads();
+    if (t == 0) {
+        int m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = max(m, wmax[w]);
+        out[r] = (float)m;
+    }
+}
+
+static int engine_cluster(gasp_engine *e, int spin) {
+    const size_t n8 = (size_t)e->R * e->N;
+    if (!e->d_spins8) HIP_TRY(hipMalloc(&e->d_spins8, n8));
+    if (!e->d_lclus) HIP_TRY(hipMalloc(&e->d_lclus, (size_t)e->R * sizeof(float)));
+    int rc = engine_unpack<int8_t>(e, e->d_spins8);
+    if (rc) return rc;
+    DeviceInfo di = device_info();
+    const size_t lds = 8 * (size_t)e->N;
+    if (lds + 64 <= (di.ok ? di.lds_block : 65536)) {
+        if (lds > 65536)
+            HIP_TRY(hipFuncSetAttribute((const void *)cluster_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(cluster_kernel<true>, dim3(e->R), dim3(256), lds, e->stream, e->d_spins8, e->N, e->L,
+                           spin, (int *)nullptr, e->d_lclus);
+    } else {
+        int *work = nullptr;
+        HIP_TRY(hipMallocAsync((void **)&work, (size_t)e->R * 2 * e->N * sizeof(int), e->stream));
+        hipLaunchKernelGGL(cluster_kernel<false>, dim3(e->R), dim3(256), 0, e->stream, e->d_spins8, e->N, e->L,
+                           spin, work, e->d_lclus);
+        HIP_TRY(hipFreeAsync(work, e->stream));
+    }
+    HIP_TRY(hipGetLastError());
+    return GASP_OK;
+}
+
+static int engine_check_cv(gasp_engine *e, const float *d_cv, int check, double dn, double up) {
+    HIP_TRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
+    hipLaunchKernelGGL(check_cv_kernel, dim3((e->R + 255) / 256), dim3(256), 0, e->stream, d_cv, e->R, check,
+                       dn, up, e->d_fate, e->d_counts);
+    HIP_TRY(hipGetLastError());
+    return GASP_OK;
+}
+
+static int engine_store_rng(gasp_engine *e) {
+    HIP_TRY(hipMemcpyAsync(e->d_rng, &e->rng, sizeof(RngHeader), hipMemcpyHostToDevice, e->stream));
+    return engine_sync(e);
+}
+
+// ------------------------------------------------------------ engine C API
+extern "C" int gasp_engine_create(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta,
+                                  double h, gasp_engine **out) {
+    clear_error();
+    if (!out) return set_error(GASP_ERR_ARG, "gasp_engine_create: NULL output pointer");
+    float pf[20];
+    uint64_t thr[10];
+    table_f32(beta, h, pf);
+    thr_from_table_f(pf, thr);
+    return engine_new(L, ngrids, gpu_method, d_state, thr, 0, beta, h, out);
+}
+
+extern "C" int gasp_engine_destroy(gasp_engine *e) {
+    if (!e) return GASP_OK;
+    int rc = engine_store_rng(e);
+    engine_free(e);
+    return rc;
+}
+
+extern "C" int gasp_engine_set_grids(gasp_engine *e, const int *grids, int on_device) {
+    clear_error();
+    if (!e || !grids) return set_error(GASP_ERR_ARG, "gasp_engine_set_grids: NULL argument");
+    int rc = engine_set_grids_impl(e, grids, on_device);
+    return rc ? rc : engine_sync(e);
+}
+
+extern "C" int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device) {
+    clear_error();
+    if (!e || !grids) return set_error(GASP_ERR_ARG, "gasp_engine_get_grids: NULL argument");
+    int rc = engine_get_grids_impl(e, grids, on_device);
+    return rc ? rc : engine_sync(e);
+}
+
+extern "C" int gasp_engine_sweep(gasp_engine *e, int nsweeps, int check, double dn_thr, double up_thr) {
+    clear_error();
+    if (!e) return set_error(GASP_ERR_ARG, "gasp_engine_sweep: NULL engine");
+    if (nsweeps < 0 || check < 0 || check > 2) return set_error(GASP_ERR_ARG, "gasp_engine_sweep: bad argument");
+    return engine_advance(e, (uint64_t)nsweeps * (uint64_t)e->N, check, dn_thr, up_thr);
+}
+
+extern "C" int gasp_engine_sync(gasp_engine *e) {
+    if (!e) return set_error(GASP_ERR_ARG, "gasp_engine_sync: NULL engine");
+    return engine_sync(e);
+}
+
+extern "C" int gasp_engine_get_magnetisation(gasp_engine *e, float *host_out) {
+    if (!e || !host_out) return set_error(GASP_ERR_ARG, "gasp_engine_get_magnetisation: NULL argument");
+    HIP_TRY(hipMemcpyAsync(host_out, e->d_mag, (size_t)e->R * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    return engine_sync(e);
+}
+
+extern "C" int gasp_engine_get_counts(gasp_engine *e, int *counts3) {
+    if (!e || !counts3) return set_error(GASP_ERR_ARG, "gasp_engine_get_counts: NULL argument");
+    HIP_TRY(hipMemcpyAsync(counts3, e->d_counts, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    return engine_sync(e);
+}
+
+extern "C" int gasp_engine_set_fates(gasp_engine *e, const int *host_fates) {
+    if (!e || !host_fates) return set_error(GASP_ERR_ARG, "gasp_engine_set_fates: NULL argument");
+    HIP_TRY(hipMemcpyAsync(e->d_fate, host_fates, (size_t)e->R * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    return engine_sync(e);
+}
+
+extern "C" int gasp_engine_get_fates(gasp_engine *e, int *host_fates) {
+    if (!e || !host_fates) return set_error(GASP_ERR_ARG, "gasp_engine_get_fates: NULL argument");
+    HIP_TRY(hipMemcpyAsync(host_fates, e->d_fate, (size_t)e->R * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    return engine_sync(e);
+}
+
+extern "C" int gasp_engine_largest_cluster(gasp_engine *e, int spin, float *host_out) {
+    clear_error();
+    if (!e || !host_out) return set_error(GASP_ERR_ARG, "gasp_engine_largest_cluster: NULL argument");
+    int rc = engine_cluster(e, spin);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(host_out, e->d_lclus, (size_t)e->R * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    return engine_sync(e);
+}
+
+extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
+    if (!e) return 0.0;
+    engine_sync(e);
+    return e->kernel_ms;
+}
+
+extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+// ============================================================ device set-up
+extern "C" int gpuInitDevice(int deviceIndex, int *gpu_nsms) {
+    clear_error();
+    int count = 0;
+    hipError_t err = hipGetDeviceCount(&count);
+    if (count == 0 || err != hipSuccess) {
+        emit("No HIP supported devices are available in this system.\n");
+        set_error(GASP_ERR_NODEV, "gpuInitDevice: no GPU visible");
+        return -1;
+    }
+    emit("Found %d HIP devices in this system\n", count);
+    for (int idev = 0; idev < count; ++idev) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, idev) != hipSuccess) continue;
+        emit("Device %d : %s (%s)\n", idev, prop.name, prop.gcnArchName);
+        emit("================================\n");
+        emit("Number of CUs       : %d\n", prop.multiProcessorCount);
+        emit("Max SHMEM per block : %ld KB\n", (long)(prop.sharedMemPerBlock / 1024));
+        emit("Max LDS per CU      : %ld KB\n", (long)(prop.maxSharedMemoryPerMultiProcessor / 1024));
+        emit("Recommended ngrids  : %d\n", 4 * prop.warpSize * prop.multiProcessorCount);
+        emit("\n");
+        const long avail = (long)prop.sharedMemPerBlock - kLdsLattice;
+        const int maxL = avail > 0 ? (int)floor(sqrt((double)avail / 8.0)) : 0;
+        emit("Estimated largest L for shared memory code path : %d\n", maxL);
+        emit("(64 replicas per 64-bit multispin word)\n");
+        emit("\n");
+    }
+    if (deviceIndex != -1) {
+        err = hipSetDevice(deviceIndex);
+        if (err != hipSuccess) {
+            set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(err), __FILE__, __LINE__);
+            return -1;
+        }
+    }
+    int idev = 0;
+    if (hipGetDevice(&idev) != hipSuccess) return -1;
+    emit("Using HIP device : %d\n", idev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, idev) == hipSuccess && gpu_nsms) *gpu_nsms = prop.multiProcessorCount;
+    return idev;
+}
+
+extern "C" void gpuInitGrid(int L, int ngrids, int threadsPerBlock, int *ising_grids, int **d_ising_grids,
+                            int **d_neighbour_list) {
+    (void)threadsPerBlock;
+    clear_error();
+    if (!d_ising_grids || !d_neighbour_list || !ising_grids || L < 1 || ngrids < 1) {
+        set_error(GASP_ERR_ARG, "gpuInitGrid: bad argument");
+        return;
+    }
+    *d_ising_grids = nullptr;
+    *d_neighbour_list = nullptr;
+    const size_t n = (size_t)L * L * ngrids;
+    hipError_t e = hipMalloc((void **)d_ising_grids, n * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(*d_ising_grids, ising_grids, n * sizeof(int), hipMemcpyHostToDevice);
+    // neighbour table of src/mc_gpu.cu:39-58 (down, up, right, left per site)
+    std::vector<int> nb((size_t)4 * L * L);
+    for (int s = 0; s < L * L; ++s) {
+        const int row = s / L, col = s % L;
+        nb[4 * s + 0] = L * ((row + 1) % L) + col;
+        nb[4 * s + 1] = L * ((row + L - 1) % L) + col;
+        nb[4 * s + 2] = L * row + (col + 1) % L;
+        nb[4 * s + 3] = L * row + (col + L - 1) % L;
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)d_neighbour_list, nb.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(*d_neighbour_list, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(e), __FILE__, __LINE__);
+}
+
+extern "C" void preComputeProbs_gpu(double beta, double h) {
+    table_f32(beta, h, g_pacc_f32);
+    thr_from_table_f(g_pacc_f32, g_thr_f32);
+}
+
+extern "C" int select_gpu_method(int L, int ngrids, int threadsPerBlock, int gpu_device) {
+    (void)ngrids;
+    (void)threadsPerBlock;
+    hipDeviceProp_t prop;
+    size_t lds = 65536;
+    if (hipGetDeviceProperties(&prop, gpu_device < 0 ? 0 : gpu_device) == hipSuccess) lds = prop.sharedMemPerBlock;
+    const int N = L * L;
+    if (lds_bytes_for(2, N, 1, 1) <= lds) return 2;
+    if ((size_t)kLdsLattice + (size_t)layout_stride(1, N) * 4 <= lds) return 1;
+    return 0;
+}
+
+// ============================================================ driver
+// Schedule of src/mc_gpu.cu:654-778.  Per iteration (isweep a multiple of
+// spc = min(mag, grid)): grids to the host if isweep % grid == 0; the
+// collective variable of the state at isweep (computed in the epilogue of the
+// previous sweep launch, or by the initial measurement) if isweep % mag == 0;
+// the next sweep launch; the output callback; the threshold report, which
+// may stop the run.  The reference launches one more sweep before it stops;
+// that sweep is not observable through the API and is skipped here.
+extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *grid_fate, mc_sampler_t samples,
+                              mc_function_t calc, gpu_run_t gpu_state, GridOutputFunc outfunc) {
+    clear_error();
+    const int L = grids.L, R = grids.ngrids;
+    const int tot = samples.tot_nsweeps, mag = samples.mag_output_int, gout = samples.grid_output_int;
+    const int itask = calc.itask;
+    const double dn = calc.dn_thr, up = calc.up_thr;
+    const int ninputs = calc.ninputs;
+    if (ninputs < 1 || R % ninputs != 0) {
+        set_error(GASP_ERR_ARG, "Error: ngrids must be divisible by ninputs!");
+        return;
+    }
+    if (mag < 1 || gout < 1) {
+        set_error(GASP_ERR_ARG, "Error: mag_output_int and grid_output_int must be positive");
+        return;
+    }
+    if (itask != 0 && itask != 1) {
+        set_error(GASP_ERR_ARG, "Error: itask must be 0 or 1!");
+        return;
+    }
+    if (gpu_state.gpu_method < 0 || gpu_state.gpu_method > 2) {
+        set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
+        return;
+    }
+    if (gpu_state.gpu_method == 2 && gpu_state.threadsPerBlock != 32 && gpu_state.threadsPerBlock != 64) {
+        set_error(GASP_ERR_ARG, "Invalid threadsPerBlock for gpu_method=2");
+        return;
+    }
+    if (!grids.d_ising_grids || !grids.ising_grids || (itask == 1 && !grid_fate) || !calc.result) {
+        set_error(GASP_ERR_ARG, "Error: mc_driver_gpu received a NULL array");
+        return;
+    }
+    const bool use_cluster = calc.cv && strcmp(calc.cv, "largest_cluster") == 0;
+    const int sub = R / ninputs;
+    const int spc = mag < gout ? mag : gout;
+    const int result_size = itask == 0 ? tot / mag : ninputs;
+
+    gasp_engine *e = nullptr;
+    if (engine_new(L, R, gpu_state.gpu_method, gpu_state.d_state, g_thr_f32, 0, beta, h, &e)) return;
+
+    std::vector<float> magnetisation(R, 0.0f), lclus(use_cluster ? R : 0, 0.0f);
+    std::vector<float> result(std::max(result_size, 1), 0.0f);
+    if (itask == 0)
+        for (int i = 0; i < result_size; ++i) result[i] = 1.0f;
+    const int check_mode = itask == 0 ? CHECK_COUNT : CHECK_FATES;
+    int counts[4] = {0, 0, 0, 0};
+    int rc = engine_set_grids_impl(e, grids.d_ising_grids, 1);
+    if (!rc && itask == 1)
+        rc = hipMemcpyAsync(e->d_fate, grid_fate, (size_t)R * sizeof(int), hipMemcpyHostToDevice, e->stream) ==
+                     hipSuccess
+                 ? GASP_OK
+                 : set_error(GASP_ERR_HIP, "GPUassert: fate upload");
+    // collective variable of the initial state (isweep = 0 is always a check)
+    if (!rc) rc = launch_sweep(e, 0, nullptr, use_cluster ? CHECK_NONE : check_mode, dn, up);
+    const double t_ms0 = e->kernel_ms;
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool py = output_style() == 1;
+
+    int isweep = 0;
+    while (!rc && isweep < tot) {
+        const bool do_grid = isweep % gout == 0;
+        const bool do_mag = isweep % mag == 0;
+        if ((rc = engine_sync(e))) break;
+        if (do_mag && use_cluster) {
+            if ((rc = engine_cluster(e, -calc.initial_spin))) break;
+            if ((rc = engine_check_cv(e, e->d_lclus, check_mode, dn, up))) break;
+        }
+        if (do_grid) {
+            if ((rc = engine_unpack<int>(e, grids.d_ising_grids))) break;
+            if (hipMemcpyAsync(grids.ising_grids, grids.d_ising_grids, (size_t)R * L * L * sizeof(int),
+                               hipMemcpyDeviceToHost, e->stream) != hipSuccess) {
+                rc = set_error(GASP_ERR_HIP, "GPUassert: grid download");
+                break;
+            }
+        }
+        if (do_mag) {
+            if (hipMemcpyAsync(magnetisation.data(), e->d_mag, (size_t)R * sizeof(float), hipMemcpyDeviceToHost,
+                               e->stream) != hipSuccess ||
+                hipMemcpyAsync(counts, e->d_counts, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream) !=
+                    hipSuccess ||
+                (use_cluster && hipMemcpyAsync(lclus.data(), e->d_lclus, (size_t)R * sizeof(float),
+                                               hipMemcpyDeviceToHost, e->stream) != hipSuccess)) {
+                rc = set_error(GASP_ERR_HIP, "GPUassert: collective variable download");
+                break;
+            }
+        }
+        if ((rc = engine_sync(e))) break;
+        const bool report = do_mag || isweep == tot - 1;
+        bool stop = false;
+        if (report) stop = itask == 0 ? counts[0] == R : counts[1] + counts[2] == R;
+        if (!stop) {
+            const int next = isweep + spc;
+            const int chk = (!use_cluster && next % mag == 0) ? check_mode : CHECK_NONE;
+            if ((rc = engine_advance(e, (uint64_t)spc * (uint64_t)e->N, chk, dn, up))) break;
+        }
+        if (do_grid || isweep == tot - 1) {
+            if (outfunc)
+                outfunc(L, R, grids.ising_grids, isweep, magnetisation.data(), use_cluster ? lclus.data() : nullptr,
+                        calc.cv, dn, up, calc.filename);
+        }
+        if (report) {
+            if (itask == 0) {
+                const int nnuc = counts[0];
+                const int k = isweep / mag;
+                if (k < result_size) result[k] = (float)((double)nnuc / (double)R);
+                if (py)
+                    emit("\r Sweep : %10d, Reached cv = %6.2f : %4d , Unresolved : %4d", isweep, up, nnuc, R - nnuc);
+                else
+                    emit("%10d  %12.6f\n", isweep, (double)nnuc / (double)R);
+            } else {
+                const int nA = counts[1], nB = counts[2];
+                emit("\r Sweep : %10d, Reached cv = %6.2f : %4d , Reached cv = %6.2f : %4d , Unresolved : %4d", isweep,
+                     dn, nA, up, nB, R - nA - nB);
+            }
+            if (stop) break;
+        }
+        isweep += spc;
+    }
+    if (!rc) rc = engine_sync(e);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // final state back into the caller's device grids
+    if (!rc) rc = engine_unpack<int>(e, grids.d_ising_grids);
+    if (!rc && itask == 1)
+        rc = hipMemcpyAsync(grid_fate, e->d_fate, (size_t)R * sizeof(int), hipMemcpyDeviceToHost, e->stream) ==
+                     hipSuccess
+                 ? GASP_OK
+                 : set_error(GASP_ERR_HIP, "GPUassert: fate download");
+    if (!rc) rc = engine_store_rng(e);
+    t_stats.sweep_kernel_ms = e->kernel_ms - t_ms0;
+    t_stats.attempts_per_replica = (double)(e->rng.attempts);
+    t_stats.launches = e->launches;
+    t_stats.slots_per_replica = e->S;
+    t_stats.waves_per_group = e->W;
+    t_stats.method = e->method;
+    engine_free(e);
+    if (rc) return;
+
+    if (itask == 0) {
+        for (int i = 0; i < result_size; ++i) calc.result[i] = result[i];
+    } else {
+        // src/mc_gpu.cu:812-826
+        for (int ii = 0; ii < ninputs; ++ii) {
+            int nB = 0, nF = 0;
+            for (int jj = 0; jj < sub; ++jj) {
+                const int f = grid_fate[ii * sub + jj];
+                if (f > -1) nB += f;
+                else nF += 1;
+            }
+            calc.result[ii] = (float)nB / (float)(sub - nF);
+        }
+    }
+    if (py) {
+        emit("\n");
+    } else {
+        emit("\n# Time taken on GPU = %f seconds\n", secs);
+        if (itask == 1) emit("pB estimate : %10.6f\n", (double)calc.result[0]);
+    }
+}

@@ -1,0 +1,597 @@
+// gasp_kernels.h -- gfx950 device code of the replica-parallel Metropolis engine.
+//
+// One sweep kernel template covers the three lattice layouts of the reference's
+// gpu_method 0/1/2 (src/mc_gpu.cu:88-151, 173-275, 279-434), the two random
+// streams (counter-based Philox4x32-10, or a pre-generated per-replica MT19937
+// draw stream that reproduces src/mc_cpu.c:41-89), and a speculation width S.
+//
+// Dynamics (identical in every layout).  Attempt a of a replica picks a site,
+// reads the spin s and its four periodic neighbours, forms
+// idx = 5*(s+1) + nsum + 4 (src/mc_cpu.c:65), and flips s iff the 32-bit
+// acceptance draw u satisfies u < T[idx], where T is the exact integer image of
+// the reference's acceptance table (xi = (u+0.5)*2^-32 < P  <=>  u < T).
+//
+// Speculation.  A replica is a sequential Markov chain.  S lanes of one wave
+// serve one replica and each lane evaluates two consecutive attempts (an
+// attempt pair shares one Philox block), so one round evaluates 2*S attempts
+// a0 .. a0+2S-1 against the current lattice.  The attempts are then committed
+// in order up to (excluding) the first one whose 5-site stencil contains the
+// site of an earlier accepted attempt of the same round: every committed
+// attempt saw exactly the lattice the sequential chain would have shown it,
+// so the trajectory is the sequential one, bit for bit.  The first
+// conflicting attempt is redone in the next round (counter-based draws make
+// that free).  S = 1 is the plain one-lane-per-replica chain.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gasp {
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+constexpr uint32_t kPhiloxDomain = 0x47415350u;  // "GASP"
+
+enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2 };
+enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
+enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
+
+// LDS header shared by all layouts: 10 thresholds (80 B, padded to 128) and
+// 64 per-replica counters.
+constexpr int kLdsThr = 0;
+constexpr int kLdsCnt = 128;
+constexpr int kLdsLattice = 384;
+
+struct SweepArgs {
+    void *lattice;           // native layout (see engine)
+    int L, N, R;
+    int rpw;                 // replicas per workgroup (layouts 0/1)
+    int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: words per group
+    uint64_t a_begin;        // per-replica attempt index of the first attempt
+    uint64_t n_attempts;     // attempts per replica in this launch
+    uint64_t seed;           // Philox key
+    const uint32_t *draws;   // RNG_DRAWS: [R][3*n_attempts], draw stream from a_begin
+    const uint64_t *thr;     // 10 thresholds, j = 5*s01 + (number of up neighbours)
+    float *mag;              // out: magnetisation per replica
+    int *fate;               // CHECK_FATES: fates per replica
+    int *counts;             // out: [0] #(m>up), [1] #fate 0, [2] #fate 1
+    int check;
+    double dn_thr, up_thr;
+};
+
+__device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32_t &c2,
+                                              uint32_t &c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+}
+
+// Exact site maps.
+//  Philox: site = floor(w*N / 2^32) with row = floor(w*L / 2^32) and col from
+//  the fractional part, which is the same integer.
+//  MT19937: row = floor(L * (u+0.5)*2^-32) = ((2u+1)*L) >> 33 (src/mc_cpu.c:51).
+__device__ __forceinline__ void philox_site(uint32_t w, uint32_t L, int &row, int &col) {
+    const uint64_t t = (uint64_t)w * L;
+    row = (int)(t >> 32);
+    col = (int)(((uint64_t)(uint32_t)t * L) >> 32);
+}
+__device__ __forceinline__ int real3_floor(uint32_t u, uint32_t L) {
+    return (int)((((uint64_t)u << 1) + 1ull) * L >> 33);
+}
+
+struct Stencil {
+    int c, up, dn, lt, rt;
+};
+__device__ __forceinline__ Stencil make_stencil(int row, int col, int L, int N) {
+    Stencil s;
+    s.c = row * L + col;
+    s.up = (row == 0) ? s.c + (N - L) : s.c - L;
+    s.dn = (row == L - 1) ? s.c - (N - L) : s.c + L;
+    s.lt = (col == 0) ? s.c + (L - 1) : s.c - 1;
+    s.rt = (col == L - 1) ? s.c - (L - 1) : s.c + 1;
+    return s;
+}
+__device__ __forceinline__ bool in_stencil(int x, const Stencil &s) {
+    return (x == s.c) | (x == s.up) | (x == s.dn) | (x == s.lt) | (x == s.rt);
+}
+
+// ------------------------------------------------------------ lattice access
+// Layout 2 (multispin): uint64 word per site, bit b = replica (64*group + b).
+//   Lane reads the byte holding its bit; flips with an LDS atomic XOR on the
+//   dword holding it (other waves flip other bits of the same word).
+// Layout 1 (bit lattice): replica-private region of `stride` dwords in LDS,
+//   bit (site & 31) of dword (site >> 5).
+// Layout 0 (HBM): one byte (0/1) per site in global memory, replica stride
+//   `stride` bytes; accessed with workgroup-scope atomics so that the lanes of
+//   a replica see each other's flips.
+template <int LAT>
+struct Lattice {
+    unsigned char *base;  // LDS (1/2) or global (0) base of this lane's replica
+    unsigned *dw;         // layout 2: LDS lattice as dwords
+    int rl;               // layout 2: replica bit within the 64-bit word
+
+    __device__ __forceinline__ int spin(int site) const {
+        if constexpr (LAT == LAT_MULTISPIN) {
+            return (base[site * 8] >> (rl & 7)) & 1;
+        } else if constexpr (LAT == LAT_BITS) {
+            return (base[site >> 3] >> (site & 7)) & 1;
+        } else {
+            return __hip_atomic_load(base + site, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __device__ __forceinline__ void flip(int site, int old) const {
+        if constexpr (LAT == LAT_MULTISPIN) {
+            // word `site` is dwords 2*site (bits 0..31) and 2*site+1 (bits 32..63)
+            atomicXor(dw + 2 * site + (rl >> 5), 1u << (rl & 31));
+        } else if constexpr (LAT == LAT_BITS) {
+            atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
+        } else {
+            __hip_atomic_store(base + site, (unsigned char)(old ^ 1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+};
+
+struct Attempt {
+    Stencil st;
+    int s01;
+    bool acc;
+};
+
+template <int LAT>
+__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, int row, int col, int L,
+                                            int N, uint32_t ua, bool en,
+                                            const uint64_t *thr_lds) {
+    Attempt t;
+    t.st = make_stencil(row, col, L, N);
+    t.s01 = lat.spin(t.st.c);
+    const int nup = lat.spin(t.st.up) + lat.spin(t.st.dn) + lat.spin(t.st.lt) + lat.spin(t.st.rt);
+    const uint64_t T = thr_lds[5 * t.s01 + nup];
+    t.acc = en && ((uint64_t)ua < T);
+    return t;
+}
+
+// Per-replica epilogue: magnetisation and the threshold check.  Must be
+// called by a whole wave (ballots); `own` marks the lane reporting replica r.
+__device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int r,
+                                               unsigned nup_count) {
+    float m = 0.0f;
+    bool gt_up = false, is_a = false, is_b = false;
+    if (own) {
+        const int sum = 2 * (int)nup_count - a.N;
+        // src/mc_gpu.cu:445-449: float sum of +-1 spins (exact) over float N
+        m = (float)sum / (float)a.N;
+        a.mag[r] = m;
+        gt_up = (double)m > a.up_thr;
+        if (a.check == CHECK_FATES) {
+            int f = a.fate[r];
+            if (f == -1) {
+                if ((double)m > a.up_thr) f = 1;
+                else if ((double)m < a.dn_thr) f = 0;
+                if (f != -1) a.fate[r] = f;
+            }
+            is_a = (f == 0);
+            is_b = (f == 1);
+        }
+    }
+    if (a.check == CHECK_COUNT) {
+        const uint64_t b = __ballot(gt_up);
+        if ((threadIdx.x & 63) == 0 && b) atomicAdd(&a.counts[0], (int)__popcll(b));
+    } else if (a.check == CHECK_FATES) {
+        const uint64_t ba = __ballot(is_a), bb = __ballot(is_b);
+        if ((threadIdx.x & 63) == 0) {
+            if (ba) atomicAdd(&a.counts[1], (int)__popcll(ba));
+            if (bb) atomicAdd(&a.counts[2], (int)__popcll(bb));
+        }
+    }
+}
+
+template <int LAT, int RNG, int S>
+__global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
+    static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
+    unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);
+    unsigned char *lat_lds = smem + kLdsLattice;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int L = a.L, N = a.N;
+    const int slot = lane & (S - 1);
+    const int gbase = lane & ~(S - 1);
+
+    int rl, r0;
+    if constexpr (LAT == LAT_MULTISPIN) {
+        rl = wave * (64 / S) + (lane / S);
+        r0 = blockIdx.x * 64;
+    } else {
+        rl = tid / S;
+        r0 = blockIdx.x * a.rpw;
+    }
+    const int r = r0 + rl;
+    const bool active = r < a.R;
+
+    // ---------------------------------------------------------- prologue
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+        if (tid == i) thr_lds[i] = a.thr[i];
+    if (tid < 64) cnt_lds[tid] = 0u;
+    if constexpr (LAT == LAT_MULTISPIN) {
+        // a.stride words per group (N rounded up to even): 16-byte aligned
+        const uint4 *src = (const uint4 *)((const uint64_t *)a.lattice + (size_t)blockIdx.x * a.stride);
+        uint4 *dst = (uint4 *)lat_lds;
+        const int n16 = a.stride / 2;
+        for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+    } else if constexpr (LAT == LAT_BITS) {
+        const int nrep = min(a.rpw, a.R - r0);
+        const int ndw = nrep * a.stride;
+        const unsigned *src = (const unsigned *)a.lattice + (size_t)r0 * a.stride;
+        unsigned *dst = (unsigned *)lat_lds;
+        for (int i = tid; i < ndw; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+
+    Lattice<LAT> lat;
+    lat.dw = (unsigned *)lat_lds;
+    lat.rl = rl;
+    if constexpr (LAT == LAT_MULTISPIN) {
+        lat.base = lat_lds + (rl >> 3);
+    } else if constexpr (LAT == LAT_BITS) {
+        lat.base = lat_lds + (size_t)rl * a.stride * 4;
+    } else {
+        lat.base = (unsigned char *)a.lattice + (size_t)(active ? r : 0) * a.stride;
+    }
+
+    const uint64_t a_end = a.a_begin + a.n_attempts;
+    uint64_t a0 = active ? a.a_begin : a_end;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t *drw = nullptr;
+    if constexpr (RNG == RNG_DRAWS)
+        drw = a.draws + (size_t)(active ? r : 0) * (3 * a.n_attempts);
+
+    // ---------------------------------------------------------- rounds
+    while (__ballot(a0 < a_end) != 0ull) {
+        const uint64_t p0 = a0 >> 1;
+        const uint64_t p = p0 + (uint64_t)slot;
+        const uint64_t attA = 2 * p, attB = attA + 1;
+        const bool enA = (attA >= a0) && (attA < a_end);
+        const bool enB = (attB >= a0) && (attB < a_end);
+
+        int rowA, colA, rowB, colB;
+        uint32_t uaA, uaB;
+        if constexpr (RNG == RNG_PHILOX) {
+            uint32_t c0 = (uint32_t)p, c1 = (uint32_t)(p >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
+            philox4x32_10(c0, c1, c2, c3, k0, k1);
+            philox_site(c0, (uint32_t)L, rowA, colA);
+            uaA = c1;
+            philox_site(c2, (uint32_t)L, rowB, colB);
+            uaB = c3;
+        } else {
+            uint32_t u0 = 0, u1 = 0, u2 = 0, v0 = 0, v1 = 0, v2 = 0;
+            if (enA) {
+                const uint32_t *d = drw + 3 * (attA - a.a_begin);
+                u0 = d[0]; u1 = d[1]; u2 = d[2];
+            }
+            if (enB) {
+                const uint32_t *d = drw + 3 * (attB - a.a_begin);
+                v0 = d[0]; v1 = d[1]; v2 = d[2];
+            }
+            rowA = real3_floor(u0, (uint32_t)L);
+            colA = real3_floor(u1, (uint32_t)L);
+            uaA = u2;
+            rowB = real3_floor(v0, (uint32_t)L);
+            colB = real3_floor(v1, (uint32_t)L);
+            uaB = v2;
+        }
+
+        const Attempt A = evaluate<LAT>(lat, rowA, colA, L, N, uaA, enA, thr_lds);
+        const Attempt B = evaluate<LAT>(lat, rowB, colB, L, N, uaB, enB, thr_lds);
+
+        bool conflA = false;
+        bool conflB = A.acc && in_stencil(A.st.c, B.st);
+        bool validA, validB;
+        if constexpr (S > 1) {
+            const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
+            uint64_t m = mA | mB;
+            while (m) {
+                const int i = __builtin_ctzll(m);
+                m &= m - 1;
+                const bool later = ((lane ^ i) < S) && (lane > i);
+                if ((mA >> i) & 1ull) {
+                    const int x = __builtin_amdgcn_readlane(A.st.c, i);
+                    conflA |= later && in_stencil(x, A.st);
+                    conflB |= later && in_stencil(x, B.st);
+                }
+                if ((mB >> i) & 1ull) {
+                    const int x = __builtin_amdgcn_readlane(B.st.c, i);
+                    conflA |= later && in_stencil(x, A.st);
+                    conflB |= later && in_stencil(x, B.st);
+                }
+            }
+            const uint64_t cA = __ballot(conflA), cB = __ballot(conflB);
+            const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << gbase) - 1ull);
+            validA = (((cA | cB) & below) == 0ull) && !conflA;
+            validB = validA && !conflB;
+        } else {
+            validA = true;
+            validB = !conflB;
+        }
+
+        if (A.acc && validA) lat.flip(A.st.c, A.s01);
+        if (B.acc && validB) lat.flip(B.st.c, B.s01);
+
+        const bool invA = enA && !validA, invB = enB && !validB;
+        if constexpr (S > 1) {
+            const uint64_t iA = __ballot(invA), iB = __ballot(invB);
+            const uint64_t gm = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << gbase);
+            const uint64_t x = (iA | iB) & gm;
+            if (x == 0ull) {
+                const uint64_t nxt = 2 * (p0 + S);
+                a0 = nxt < a_end ? nxt : a_end;
+            } else {
+                const int f = __builtin_ctzll(x);
+                a0 = 2 * (p0 + (uint64_t)(f - gbase)) + (((iA >> f) & 1ull) ? 0ull : 1ull);
+            }
+            if constexpr (LAT == LAT_HBM) {
+                // lanes of a replica must see each other's byte stores next round
+                if (__ballot((A.acc && validA) || (B.acc && validB)) != 0ull) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
+            }
+        } else {
+            (void)invA;
+            if (invB) {
+                a0 = attB;
+            } else {
+                const uint64_t nxt = 2 * p0 + 2;
+                a0 = nxt < a_end ? nxt : a_end;
+            }
+        }
+    }
+
+    // ---------------------------------------------------------- epilogue
+    if constexpr (LAT == LAT_MULTISPIN) {
+        __syncthreads();
+        // thread t counts bit (t & 63) over sites t>>6, t>>6 + S, ...
+        const int b = tid & 63;
+        const int q = tid >> 6;
+        const uint64_t *w = (const uint64_t *)lat_lds;
+        unsigned c = 0;
+        for (int site = q; site < N; site += S) c += (unsigned)((w[site] >> b) & 1ull);
+        atomicAdd(&cnt_lds[b], c);
+        __syncthreads();
+        if (tid < 64) finish_replica(a, (r0 + tid) < a.R, r0 + tid, cnt_lds[tid]);
+        uint4 *dst = (uint4 *)((uint64_t *)a.lattice + (size_t)blockIdx.x * a.stride);
+        const uint4 *src = (const uint4 *)lat_lds;
+        const int n16 = a.stride / 2;
+        for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+    } else {
+        if constexpr (LAT == LAT_BITS) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        unsigned c = 0;
+        if (active) {
+            if constexpr (LAT == LAT_BITS) {
+                const unsigned *w = (const unsigned *)lat.base;
+                for (int i = slot; i < a.stride; i += S) c += __popc(w[i]);
+            } else {
+                const unsigned *w = (const unsigned *)lat.base;
+                for (int i = slot; i < a.stride / 4; i += S)
+                    c += __popc(__hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                                0x01010101u);
+            }
+        }
+#pragma unroll
+        for (int off = S / 2; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+        finish_replica(a, active && slot == 0, r, c);
+        if constexpr (LAT == LAT_BITS) {
+            const int nrep = min(a.rpw, a.R - r0);
+            const int ndw = nrep * a.stride;
+            unsigned *dst = (unsigned *)a.lattice + (size_t)r0 * a.stride;
+            const unsigned *src = (const unsigned *)lat_lds;
+            for (int i = tid; i < ndw; i += blockDim.x) dst[i] = src[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------ layout kernels
+// int32 +-1 grids [R][N] <-> native layouts.  1-D grid-stride loops over
+// (replica or group, element) so that any R fits the launch limits.
+__global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict__ dst, int R, int N, int stride) {
+    const int G = (R + 63) / 64;
+    const size_t total = (size_t)G * N;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int g = (int)(i / N), site = (int)(i - (size_t)g * N);
+        uint64_t w = 0;
+        const int nb = min(64, R - 64 * g);
+        for (int b = 0; b < nb; ++b)
+            if (src[(size_t)(64 * g + b) * N + site] == 1) w |= 1ull << b;
+        dst[(size_t)g * stride + site] = w;
+    }
+}
+
+template <typename T>
+__global__ void unpack_multispin(const uint64_t *__restrict__ src, T *__restrict__ dst, int R, int N, int stride) {
+    const int G = (R + 63) / 64;
+    const size_t total = (size_t)G * N;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int g = (int)(i / N), site = (int)(i - (size_t)g * N);
+        const uint64_t w = src[(size_t)g * stride + site];
+        const int nb = min(64, R - 64 * g);
+        for (int b = 0; b < nb; ++b) dst[(size_t)(64 * g + b) * N + site] = ((w >> b) & 1ull) ? T(1) : T(-1);
+    }
+}
+
+__global__ void pack_bits(const int *__restrict__ src, unsigned *__restrict__ dst, int R, int N, int stride) {
+    const size_t total = (size_t)R * stride;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / stride), w = (int)(i - (size_t)r * stride);
+        unsigned v = 0;
+        for (int k = 0; k < 32; ++k) {
+            const int site = 32 * w + k;
+            if (site < N && src[(size_t)r * N + site] == 1) v |= 1u << k;
+        }
+        dst[i] = v;
+    }
+}
+
+template <typename T>
+__global__ void unpack_bits(const unsigned *__restrict__ src, T *__restrict__ dst, int R, int N, int stride) {
+    const size_t total = (size_t)R * N;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / N), site = (int)(i - (size_t)r * N);
+        const unsigned v = src[(size_t)r * stride + (site >> 5)];
+        dst[i] = ((v >> (site & 31)) & 1u) ? T(1) : T(-1);
+    }
+}
+
+__global__ void pack_hbm(const int *__restrict__ src, unsigned char *__restrict__ dst, int R, int N, int stride) {
+    const size_t total = (size_t)R * stride;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / stride), site = (int)(i - (size_t)r * stride);
+        dst[i] = (site < N && src[(size_t)r * N + site] == 1) ? 1 : 0;
+    }
+}
+
+template <typename T>
+__global__ void unpack_hbm(const unsigned char *__restrict__ src, T *__restrict__ dst, int R, int N, int stride) {
+    const size_t total = (size_t)R * N;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / N), site = (int)(i - (size_t)r * N);
+        dst[i] = src[(size_t)r * stride + site] ? T(1) : T(-1);
+    }
+}
+
+// ------------------------------------------------------------ MT19937 draws
+// Per-replica states interleaved [624][R] (word k of replica r at k*R + r).
+__global__ void mt_seed_kernel(uint32_t *st, int R, uint64_t seed_base) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    uint32_t x = (uint32_t)(seed_base + (uint64_t)r);
+    st[r] = x;
+    for (int k = 1; k < 624; ++k) {
+        x = 1812433253u * (x ^ (x >> 30)) + (uint32_t)k;
+        st[(size_t)k * R + r] = x;
+    }
+}
+
+// One wave per replica: the 624-word state lives in LDS; each twist is done in
+// four dependent phases (words 0..226 use only old words; 227..453 use new
+// words 0..226; 454..622 use new words 227..395; 623 uses new words 0 and 396),
+// then the block is tempered and written to the replica's draw row.
+// Draws [q0, q0 + ndraws) of the replica's stream go to out[r*ndraws + i].
+__global__ void __launch_bounds__(64) mt_draws_kernel(uint32_t *st, int R, uint64_t q0, uint64_t ndraws,
+                                                      uint32_t *out) {
+    __shared__ uint32_t mt[624];
+    const int r = blockIdx.x;
+    const int t = threadIdx.x;
+    for (int k = t; k < 624; k += 64) mt[k] = st[(size_t)k * R + r];
+    __syncthreads();
+    // position within the current block: after q draws from a fresh seed the
+    // reference's mti is 624 when q == 0, else ((q-1) % 624) + 1
+    int mti = (q0 == 0) ? 624 : (int)((q0 - 1) % 624) + 1;
+    uint64_t done = 0;
+    uint32_t *o = out + (size_t)r * ndraws;
+    auto twist_word = [&](int k, uint32_t next, uint32_t far) {
+        const uint32_t y = (mt[k] & 0x80000000u) | (next & 0x7fffffffu);
+        return far ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    };
+    while (done < ndraws) {
+        if (mti >= 624) {
+            // phase 1: k in [0, 227)
+            for (int base = 0; base < 227; base += 64) {
+                const int k = base + t;
+                uint32_t nv = 0;
+                if (k < 227) nv = twist_word(k, mt[k + 1], mt[k + 397]);
+                __syncthreads();
+                if (k < 227) mt[k] = nv;
+                __syncthreads();
+            }
+            // phase 2: k in [227, 454)
+            for (int base = 227; base < 454; base += 64) {
+                const int k = base + t;
+                uint32_t nv = 0;
+                if (k < 454) nv = twist_word(k, mt[k + 1], mt[k - 227]);
+                __syncthreads();
+                if (k < 454) mt[k] = nv;
+                __syncthreads();
+            }
+            // phase 3: k in [454, 623)
+            for (int base = 454; base < 623; base += 64) {
+                const int k = base + t;
+                uint32_t nv = 0;
+                if (k < 623) nv = twist_word(k, mt[k + 1], mt[k - 227]);
+                __syncthreads();
+                if (k < 623) mt[k] = nv;
+                __syncthreads();
+            }
+            if (t == 0) mt[623] = twist_word(623, mt[0], mt[396]);
+            __syncthreads();
+            mti = 0;
+        }
+        // temper and emit min(624 - mti, remaining) draws
+        const uint64_t take64 = (uint64_t)(624 - mti) < (ndraws - done) ? (uint64_t)(624 - mti) : (ndraws - done);
+        const int take = (int)take64;
+        for (int i = t; i < take; i += 64) {
+            uint32_t y = mt[mti + i];
+            y ^= (y >> 11);
+            y ^= (y << 7) & 0x9d2c5680u;
+            y ^= (y << 15) & 0xefc60000u;
+            y ^= (y >> 18);
+            o[done + i] = y;
+        }
+        mti += take;
+        done += take64;
+        __syncthreads();
+    }
+    for (int k = t; k < 624; k += 64) st[(size_t)k * R + r] = mt[k];
+}
+
+// ------------------------------------------------------------ checks
+// Threshold check on an arbitrary per-replica CV array (largest-cluster CV).
+__global__ void check_cv_kernel(const float *cv, int R, int check, double dn_thr, double up_thr,
+                                int *fate, int *counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool own = r < R;
+    bool gt_up = false, is_a = false, is_b = false;
+    if (own) {
+        const double v = (double)cv[r];
+        gt_up = v > up_thr;
+        if (check == CHECK_FATES) {
+            int f = fate[r];
+            if (f == -1) {
+                if (v > up_thr) f = 1;
+                else if (v < dn_thr) f = 0;
+                if (f != -1) fate[r] = f;
+            }
+            is_a = f == 0;
+            is_b = f == 1;
+        }
+    }
+    const uint64_t b0 = __ballot(gt_up), ba = __ballot(is_a), bb = __ballot(is_b);
+    if ((threadIdx.x & 63) == 0) {
+        if (check == CHECK_COUNT && b0) atomicAdd(&counts[0], (int)__popcll(b0));
+        if (check == CHECK_FATES) {
+            if (ba) atomicAdd(&counts[1], (int)__popcll(ba));
+            if (bb) atomicAdd(&counts[2], (int)__popcll(bb));
+        }
+    }
+}
+
+}  // namespace gasp

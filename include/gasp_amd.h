@@ -1,0 +1,225 @@
+/*
+ * gasp_amd.h -- C ABI of the MI355X (gfx950) replica-parallel Metropolis engine.
+ *
+ * Drop-in boundary for GASP's GPU engine.  The types and the entry points in
+ * the first two sections keep the reference's names, argument lists and
+ * meaning, so that the reference's callers (the `gasp` CLI main and the PyGASP
+ * extension) bind them unchanged; each cites the reference declaration it
+ * replaces (paths relative to the reference root).  They are exported with C
+ * linkage (the reference's GPU functions had C++ linkage).
+ *
+ * Differences from the reference, all deliberate:
+ *  - `curandState` is replaced by the opaque `gasp_rng_state` (per-replica
+ *    counter-based Philox4x32-10 streams, or per-replica MT19937 streams that
+ *    reproduce src/mc_cpu.c's random stream bit for bit).
+ *  - Errors never call exit(): the function prints the reference's message to
+ *    stderr, records an error code readable with gasp_last_error(), and
+ *    returns.
+ *  - There is no CPU fallback inside the library (gpuInitDevice returns -1
+ *    when no GPU is visible and the caller decides what to do).
+ *
+ * All device pointers returned by this library are hipMalloc allocations and
+ * may be released with hipFree() or gasp_free().
+ */
+#ifndef GASP_AMD_H
+#define GASP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* =================================================================== types */
+
+/* include/mc_cpu.h:5-9 */
+typedef struct {
+    int L;            /* size of each L x L grid */
+    int ngrids;       /* number of grids (replicas) */
+    int *ising_grids; /* host spins, ngrids*L*L ints of +-1, grid-major */
+} mc_grids_t;
+
+/* include/mc_cpu.h:11-15 */
+typedef struct {
+    int tot_nsweeps;     /* maximum number of sweeps */
+    int mag_output_int;  /* sweeps between collective-variable checks */
+    int grid_output_int; /* sweeps between grid output callbacks */
+} mc_sampler_t;
+
+/* include/mc_cpu.h:17-26 */
+typedef struct {
+    int itask;        /* 0: nucleated fraction vs sweep, 1: committor */
+    double dn_thr;    /* cv below which a replica has returned to A */
+    double up_thr;    /* cv above which a replica has reached B */
+    int ninputs;      /* number of distinct input grids (itask 1) */
+    int initial_spin; /* majority spin of the parent phase */
+    char *cv;         /* "magnetisation" or "largest_cluster" */
+    float *result;    /* tot/mag floats (itask 0) or ninputs floats (itask 1) */
+    char *filename;   /* passed through to the output callback */
+} mc_function_t;
+
+/* include/mc_cpu.h:31 -- called synchronously on the driver's thread. */
+typedef int (*GridOutputFunc)(int L, int ngrids, int *grid_data, int isweep,
+                              float *magnetisation, float *lclus_size, char *cv,
+                              double dn_thr, double up_thr, char *filename);
+
+/* include/mc_gpu.h:12-18 */
+typedef struct {
+    int L;
+    int ngrids;
+    int *ising_grids;      /* host copy, refreshed at every grid output */
+    int *d_ising_grids;    /* device int32 spins, ngrids*L*L */
+    int *d_neighbour_list; /* kept for ABI compatibility (4*L*L ints) */
+} mc_gpu_grids_t;
+
+/* Replaces curandState (include/mc_gpu.h:21).  Opaque device allocation. */
+typedef struct gasp_rng_state gasp_rng_state;
+
+/* include/mc_gpu.h:20-24 */
+typedef struct {
+    gasp_rng_state *d_state; /* from gpuInitRand / gasp_init_rand_mt */
+    int threadsPerBlock;     /* validated as in the reference (32 or 64 for
+                                method 2); launch shapes are chosen by the
+                                engine's tuner */
+    int gpu_method;          /* 0: HBM spins, 1: 1-bit-per-spin LDS lattice per
+                                replica, 2: 64-replica multispin LDS words */
+} gpu_run_t;
+
+/* ==================================================== reference entry points */
+
+/* include/gpu_tools.h:16 (src/gpu_tools.cu:18-99).  Prints the device table,
+ * selects deviceIndex (-1 keeps the current device), stores the CU count in
+ * *gpu_nsms and returns the device index, or -1 if no GPU is visible. */
+int gpuInitDevice(int deviceIndex, int *gpu_nsms);
+
+/* include/gpu_tools.h:19 (src/gpu_tools.cu:143-159).  Allocates the device
+ * grids (ngrids*L*L int32) and the neighbour table and copies ising_grids to
+ * the device.  The caller frees both. */
+void gpuInitGrid(int L, int ngrids, int threadsPerBlock, int *ising_grids,
+                 int **d_ising_grids, int **d_neighbour_list);
+
+/* include/gpu_tools.h:22 (src/gpu_tools.cu:162-203).  One independent
+ * counter-based stream per replica: Philox4x32-10 keyed by rngseed, counter
+ * (attempt pair, replica).  The caller frees *d_state. */
+void gpuInitRand(int ngrids, int threadsPerBlock, unsigned long rngseed,
+                 gasp_rng_state **d_state);
+
+/* include/mc_gpu.h:28 (src/mc_gpu.cu:21-37).  Builds the single-precision
+ * acceptance table expf(-beta*2s(nsum+h)) on the host, exactly as the
+ * reference, and converts it to exact integer thresholds for the kernels.
+ * Process-global, like the reference's __constant__ table. */
+void preComputeProbs_gpu(double beta, double h);
+
+/* include/gpu_tools.h:38 (src/gpu_tools.cu:205-232).  2 when a 64-replica
+ * multispin lattice (8*L*L bytes) fits in one workgroup's LDS, else 1 when a
+ * single replica's bit lattice fits, else 0. */
+int select_gpu_method(int L, int ngrids, int threadsPerBlock, int gpu_device);
+
+/* include/mc_gpu.h:47 (src/mc_gpu.cu:530-831).  Same schedule, outputs and
+ * stdout text as the reference driver: itask 0 fills calc.result[isweep/mag]
+ * with the nucleated fraction (pre-filled with 1.0) and stops when all have
+ * nucleated; itask 1 fixes grid_fate[] (0: reached A, 1: reached B, -1:
+ * unresolved) and writes pB per input grid to calc.result.  outfunc receives
+ * the host grids every grid_output_int sweeps. */
+void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *grid_fate,
+                   mc_sampler_t samples, mc_function_t calc, gpu_run_t gpu_state,
+                   GridOutputFunc outfunc);
+
+/* include/grid.h:5-10 (src/grid.c:5-87).  malloc'd; caller frees.  Return
+ * NULL (and set the error) where the reference exits. */
+int *init_grids_uniform(int L, int ngrids, int spin);
+int *init_grids_from_file(int L, int ngrids);
+int *init_grids_from_array(int L, int ngrids, int ninputs, int *array);
+int *init_fates(int ngrids);
+
+/* include/io.h:8-11 (src/io.c:4-130).  gridinput.bin reader and the
+ * gridstates.bin appender (int32 L, int32 ngrids, int32 isweep, then
+ * ceil(L*L*ngrids/8) bytes, bit k set <=> spin k is +1, LSB first). */
+void read_input_grid(int L, int ngrids, int *ising_grids);
+int write_ising_grids(int L, int ngrids, int *ising_grids, int isweep,
+                      float *magnetisation, float *lclus_size, char *cv,
+                      double dn_thr, double up_thr, char *filename);
+
+/* include/bootstrap.h:6 (src/bootstrap.c:6-63).  m resamples drawn with libc
+ * rand(); returns the population standard deviation of the resampled pB. */
+double bootstrap_errbar(int ngrids, int m, int *grid_fate_host);
+
+/* ================================================== engine-specific additions */
+/* No counterpart in the reference. */
+
+/* Error reporting.  0 = success. */
+enum {
+    GASP_OK = 0,
+    GASP_ERR_ARG = 1,     /* invalid argument (reference: exit(EXIT_FAILURE)) */
+    GASP_ERR_HIP = 2,     /* HIP runtime error */
+    GASP_ERR_NOMEM = 3,   /* host or device allocation failed */
+    GASP_ERR_IO = 4,      /* file could not be opened or read */
+    GASP_ERR_NODEV = 5    /* no GPU visible */
+};
+int gasp_last_error(void);
+const char *gasp_error_string(void);
+void gasp_clear_error(void);
+
+/* Per-replica MT19937 streams, replica r seeded (rngseed + r) & 0xffffffff,
+ * consumed three draws per attempt (row, col, acceptance) exactly as
+ * src/mc_cpu.c:51-77, with the double-precision acceptance table of
+ * src/mc_cpu.c:14-38.  Grids then match the reference CPU arithmetic bit for
+ * bit.  Returns 0 or an error code. */
+int gasp_init_rand_mt(int ngrids, unsigned long rngseed, gasp_rng_state **d_state);
+
+/* Release a device allocation made by this library. */
+void gasp_free(void *d_ptr);
+
+/* Engine tuning: slots per replica (number of lanes that speculate on
+ * consecutive attempts of one replica; 0 = automatic) and waves per workgroup
+ * for methods 0/1 (0 = automatic).  Applies to subsequent runs. */
+void gasp_set_tuning(int slots_per_replica, int waves_per_group);
+void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
+                     int *waves_per_group);
+
+/* Statistics of the most recent mc_driver_gpu / gasp_engine run on this
+ * thread: sweep-kernel time (HIP events), attempts performed per replica,
+ * kernel launches. */
+typedef struct {
+    double sweep_kernel_ms;
+    double attempts_per_replica;
+    int launches;
+    int slots_per_replica;
+    int waves_per_group;
+    int method;
+} gasp_run_stats_t;
+void gasp_get_run_stats(gasp_run_stats_t *out);
+
+/* Stream-level engine (used by benchmarks and by callers that drive the
+ * schedule themselves).  The engine owns the replicas' lattice in the
+ * method's native layout.  All calls are ordered on the engine's stream. */
+typedef struct gasp_engine gasp_engine;
+int gasp_engine_create(int L, int ngrids, int gpu_method, gasp_rng_state *d_state,
+                       double beta, double h, gasp_engine **out);
+int gasp_engine_destroy(gasp_engine *e);
+/* int32 +-1 spins, ngrids*L*L; `on_device` selects device or host memory. */
+int gasp_engine_set_grids(gasp_engine *e, const int *grids, int on_device);
+int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device);
+/* Advance every replica by nsweeps sweeps.  check: 0 none, 1 count replicas
+ * with m > up_thr, 2 update fates (sticky; 1 if m > up_thr, 0 if
+ * m < dn_thr).  The magnetisation after the sweeps is always computed. */
+int gasp_engine_sweep(gasp_engine *e, int nsweeps, int check, double dn_thr,
+                      double up_thr);
+int gasp_engine_sync(gasp_engine *e);
+int gasp_engine_get_magnetisation(gasp_engine *e, float *host_out);
+/* counts[0] = #(m > up_thr), counts[1] = #fate 0, counts[2] = #fate 1. */
+int gasp_engine_get_counts(gasp_engine *e, int *counts3);
+int gasp_engine_set_fates(gasp_engine *e, const int *host_fates);
+int gasp_engine_get_fates(gasp_engine *e, int *host_fates);
+/* Largest 4-connected periodic cluster of `spin` per replica (floats). */
+int gasp_engine_largest_cluster(gasp_engine *e, int spin, float *host_out);
+/* Milliseconds spent in sweep kernels on the engine's stream so far. */
+double gasp_engine_kernel_ms(gasp_engine *e);
+/* The engine's HIP stream (hipStream_t). */
+void *gasp_engine_stream(gasp_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GASP_AMD_H */

@@ -1,0 +1,103 @@
+"""Bit-exact parity of the HIP sweep kernels with the CPU oracle.
+
+Every lattice layout (gpu_method 0/1/2) and every speculation width S is run
+through the C ABI (gasp_engine_*) and compared spin for spin with
+  - oracle.sweep_mt_per_replica: src/mc_cpu.c:41-89 restated, per-replica
+    MT19937 streams seeded seed + r (the reference CPU stream), and
+  - oracle.sweep_philox: the production Philox4x32-10 stream.
+Random +-1 starts at beta = 0.3 accept about half the attempts, so the
+speculative rounds hit conflicts often; uniform starts at the reference's
+beta = 0.54, h = 0.07 exercise the metastable regime.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BETA, H = 0.54, 0.07
+CASES = [(2, s) for s in (1, 2, 4, 8, 16)] + [(1, s) for s in (1, 2, 8, 64)] + [(0, s) for s in (1, 4, 64)]
+
+
+def _start(kind, R, L, seed):
+    if kind == "down":
+        return -np.ones((R, L, L), dtype=np.int32)
+    rs = np.random.default_rng(seed)
+    return np.where(rs.random((R, L, L)) < 0.5, 1, -1).astype(np.int32)
+
+
+def _run_engine(G, L, R, method, S, g0, seed, mt, beta, h, chunks):
+    G.set_tuning(S, 1)
+    try:
+        rng = G.RngState(R, seed, mt=mt)
+        with G.Engine(L, R, method, rng, beta, h) as e:
+            e.set_grids(g0)
+            for n in chunks:
+                e.sweep(n)
+            out = e.get_grids()
+            mag = e.magnetisation()
+        rng.free()
+    finally:
+        G.set_tuning(0, 0)
+    return out, mag
+
+
+@pytest.mark.parametrize("method,S", CASES)
+@pytest.mark.parametrize("L", [16, 17])
+def test_mt_stream_bit_exact(gpu, engine_lib, oracle, method, S, L):
+    G, O = engine_lib, oracle
+    R, seed, beta = 96, 1234567, 0.3
+    print(f"[mt] method={method} S={S} L={L} R={R}", flush=True)
+    g0 = _start("random", R, L, seed)
+    got, mag = _run_engine(G, L, R, method, S, g0, seed, True, beta, 0.07, [3, 2])
+    ref = O.sweep_mt_per_replica(L, g0.copy(), seed, 5, beta, 0.07)
+    assert np.array_equal(got, ref)
+    ref_m = np.array([O.lib().orc_magnetisation_gpu(L, ref[r].reshape(-1)) for r in range(R)],
+                     dtype=np.float32)
+    assert np.array_equal(mag, ref_m)
+
+
+@pytest.mark.parametrize("method,S", CASES)
+@pytest.mark.parametrize("start", ["random", "down"])
+def test_philox_stream_bit_exact(gpu, engine_lib, oracle, method, S, start):
+    G, O = engine_lib, oracle
+    L, R, seed = 16, 100, 987654321
+    beta = 0.3 if start == "random" else BETA
+    print(f"[philox] method={method} S={S} start={start}", flush=True)
+    g0 = _start(start, R, L, seed)
+    got, mag = _run_engine(G, L, R, method, S, g0, seed, False, beta, H, [4, 3])
+    ref = O.sweep_philox(L, g0.copy(), seed, 0, 7 * L * L, beta, H)
+    assert np.array_equal(got, ref)
+    ref_m = (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32)
+    assert np.array_equal(mag, ref_m)
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_methods_agree_at_L64(gpu, engine_lib, oracle, method):
+    """The reference's L=64 at the default tuning: same trajectory in every layout."""
+    G, O = engine_lib, oracle
+    L, R, seed = 64, 128, 2894203475
+    print(f"[L64] method={method}", flush=True)
+    g0 = _start("down", R, L, seed)
+    got, _ = _run_engine(G, L, R, method, 0, g0, seed, False, BETA, H, [10])
+    ref = O.sweep_philox(L, g0.copy(), seed, 0, 10 * L * L, BETA, H)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("L", [8, 16, 33, 64])
+def test_largest_cluster(gpu, engine_lib, oracle, L):
+    G, O = engine_lib, oracle
+    R = 40
+    print(f"[cluster] L={L}", flush=True)
+    rs = np.random.default_rng(L)
+    p = np.linspace(0.2, 0.8, R)[:, None, None]
+    g0 = np.where(rs.random((R, L, L)) < p, 1, -1).astype(np.int32)
+    rng = G.RngState(R, 1)
+    with G.Engine(L, R, 1, rng, BETA, H) as e:
+        e.set_grids(g0)
+        got_up = e.largest_cluster(1)
+        got_dn = e.largest_cluster(-1)
+    rng.free()
+    ref_up = np.array([O.lib().orc_largest_cluster(L, g0[r].reshape(-1), 1) for r in range(R)])
+    ref_dn = np.array([O.lib().orc_largest_cluster(L, g0[r].reshape(-1), -1) for r in range(R)])
+    assert np.array_equal(got_up, ref_up)
+    assert np.array_equal(got_dn, ref_dn)
