@@ -300,10 +300,8 @@ def run_driver(L: int, grids: np.ndarray, *, beta: float, h: float, rng: RngStat
     try:
         Lb.mc_driver_gpu(g, beta, h, _ip(fates), s, f, run, cb)
         check()
-        final = np.empty((R, L * L), dtype=np.int32)
-        lib().gasp_clear_error()
         # final device grids (the driver leaves the last state in d_ising_grids)
-        import ctypes.util  # noqa: F401
+        final = np.empty((R, L * L), dtype=np.int32)
         _d2h(final, d_grids)
     finally:
         Lb.gasp_free(d_grids)
