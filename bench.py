@@ -39,6 +39,25 @@ LDS_PEAK_GBS = 256 * 128 * 2.4
 METRIC = "spin-flip attempts/sec (1 GPU, L=64) + committor pB estimates/sec at MAE≤0.05"
 
 
+def committor_inputs(n: int, L: int = 64, seed: int = 12345) -> np.ndarray:
+    """Committor shooting inputs: a -1 sea with a +1 disc of radius U[4, 14]
+    (sites within periodic distance R of a uniformly random integer centre),
+    then 5% of the sites (uniform, without replacement) flipped."""
+    rs = np.random.default_rng(seed)
+    r = np.arange(L)
+    out = np.empty((n, L, L), dtype=np.int32)
+    for k in range(n):
+        rad = rs.uniform(4.0, 14.0)
+        cy, cx = rs.integers(0, L, 2)
+        dy = np.minimum(np.abs(r - cy), L - np.abs(r - cy))[:, None]
+        dx = np.minimum(np.abs(r - cx), L - np.abs(r - cx))[None, :]
+        g = np.where(dy * dy + dx * dx <= rad * rad, 1, -1).astype(np.int32)
+        flip = rs.choice(L * L, size=int(round(0.05 * L * L)), replace=False)
+        g.reshape(-1)[flip] *= -1
+        out[k] = g
+    return out
+
+
 def fit_rate(sweeps: np.ndarray, frac: np.ndarray):
     """Least-squares fit of 1 - exp(-k (t - t0)) (the reference notebooks' model)."""
     try:

@@ -29,6 +29,7 @@ EXPORTS = (
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
+    "gasp_engine_set_inputs", "gasp_engine_set_freeze",
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
@@ -125,11 +126,13 @@ def lib() -> C.CDLL:
                  "gasp_engine_get_grids", "gasp_engine_sweep", "gasp_engine_sync",
                  "gasp_engine_get_magnetisation", "gasp_engine_get_counts",
                  "gasp_engine_set_fates", "gasp_engine_get_fates",
-                 "gasp_engine_largest_cluster"):
+                 "gasp_engine_largest_cluster", "gasp_engine_set_inputs", "gasp_engine_set_freeze"):
         getattr(L, name).restype = C.c_int
     L.gasp_engine_destroy.argtypes = [vp]
     L.gasp_engine_set_grids.argtypes = [vp, vp, C.c_int]
     L.gasp_engine_get_grids.argtypes = [vp, vp, C.c_int]
+    L.gasp_engine_set_inputs.argtypes = [vp, vp, C.c_int]
+    L.gasp_engine_set_freeze.argtypes = [vp, C.c_int]
     L.gasp_engine_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
     L.gasp_engine_sync.argtypes = [vp]
     L.gasp_engine_get_magnetisation.argtypes = [vp, fp]
@@ -196,6 +199,14 @@ class Engine:
     def set_grids(self, grids: np.ndarray) -> None:
         g = np.ascontiguousarray(grids, dtype=np.int32).reshape(self.R, self.L * self.L)
         check(lib().gasp_engine_set_grids(self.ptr, g.ctypes.data, 0))
+
+    def set_inputs(self, inputs: np.ndarray) -> None:
+        """Input grids [n, L, L]; replica r starts from input r // (R // n)."""
+        g = np.ascontiguousarray(inputs, dtype=np.int32).reshape(-1, self.L * self.L)
+        check(lib().gasp_engine_set_inputs(self.ptr, g.ctypes.data, g.shape[0]))
+
+    def set_freeze(self, on: bool) -> None:
+        check(lib().gasp_engine_set_freeze(self.ptr, 1 if on else 0))
 
     def get_grids(self) -> np.ndarray:
         out = np.empty((self.R, self.L, self.L), dtype=np.int32)

@@ -286,27 +286,41 @@ extern "C" void gasp_get_run_stats(gasp_run_stats_t *out) {
 // ============================================================ kernel table
 using SweepFn = void (*)(SweepArgs);
 
-template <int LAT, int RNG>
+template <int LAT, int RNG, bool P2>
 static SweepFn pick_sweep(int S) {
     switch (S) {
-        case 1: return sweep_kernel<LAT, RNG, 1>;
-        case 2: return sweep_kernel<LAT, RNG, 2>;
-        case 4: return sweep_kernel<LAT, RNG, 4>;
-        case 8: return sweep_kernel<LAT, RNG, 8>;
-        case 16: return sweep_kernel<LAT, RNG, 16>;
+        case 1: return sweep_kernel<LAT, RNG, 1, P2>;
+        case 2: return sweep_kernel<LAT, RNG, 2, P2>;
+        case 4: return sweep_kernel<LAT, RNG, 4, P2>;
+        case 8: return sweep_kernel<LAT, RNG, 8, P2>;
+        case 16: return sweep_kernel<LAT, RNG, 16, P2>;
         default: break;
     }
     if constexpr (LAT != LAT_MULTISPIN) {
-        if (S == 32) return sweep_kernel<LAT, RNG, 32>;
-        if (S == 64) return sweep_kernel<LAT, RNG, 64>;
+        if (S == 32) return sweep_kernel<LAT, RNG, 32, P2>;
+        if (S == 64) return sweep_kernel<LAT, RNG, 64, P2>;
     }
     return nullptr;
 }
 
-static SweepFn sweep_fn(int method, int rng, int S) {
-    if (method == 2) return rng == 0 ? pick_sweep<LAT_MULTISPIN, RNG_PHILOX>(S) : pick_sweep<LAT_MULTISPIN, RNG_DRAWS>(S);
-    if (method == 1) return rng == 0 ? pick_sweep<LAT_BITS, RNG_PHILOX>(S) : pick_sweep<LAT_BITS, RNG_DRAWS>(S);
-    return rng == 0 ? pick_sweep<LAT_HBM, RNG_PHILOX>(S) : pick_sweep<LAT_HBM, RNG_DRAWS>(S);
+template <int LAT, int RNG>
+static SweepFn pick_sweep2(int S, bool pow2) {
+    return pow2 ? pick_sweep<LAT, RNG, true>(S) : pick_sweep<LAT, RNG, false>(S);
+}
+
+static SweepFn sweep_fn(int method, int rng, int S, bool pow2) {
+    if (method == 2)
+        return rng == 0 ? pick_sweep2<LAT_MULTISPIN, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTISPIN, RNG_DRAWS>(S, pow2);
+    if (method == 1)
+        return rng == 0 ? pick_sweep2<LAT_BITS, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_BITS, RNG_DRAWS>(S, pow2);
+    return rng == 0 ? pick_sweep2<LAT_HBM, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_HBM, RNG_DRAWS>(S, pow2);
+}
+
+static bool is_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
+static int ilog2(int L) {
+    int g = 0;
+    while ((1 << g) < L) ++g;
+    return g;
 }
 
 // ============================================================ engine object
@@ -323,6 +337,8 @@ struct gasp_engine {
     uint64_t *d_thr = nullptr;
     int8_t *d_spins8 = nullptr;
     int *d_tmp32 = nullptr;
+    size_t tmp32_bytes = 0;
+    int freeze = 0;          // skip replicas whose fate is decided
     uint32_t *d_draws = nullptr;
     uint64_t draws_cap = 0;  // attempts per replica that fit in d_draws
     hipStream_t stream = nullptr;
@@ -417,7 +433,7 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
                          "limit %zu); use select_gpu_method",
                          method, need, L, (size_t)di.lds_block);
     }
-    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S);
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, is_pow2(L));
     if (!fn) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
@@ -447,6 +463,11 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
         // draw stream for up to ~512 MiB per batch (12 bytes per attempt per replica)
         const uint64_t budget = 512ull << 20;
         e->draws_cap = std::max<uint64_t>(1, budget / (12ull * (uint64_t)R));
+        // GASP_MT_BATCH_ATTEMPTS lowers the batch (exercises batch boundaries in tests)
+        if (const char *env = getenv("GASP_MT_BATCH_ATTEMPTS")) {
+            const unsigned long long v = strtoull(env, nullptr, 10);
+            if (v > 0 && v < e->draws_cap) e->draws_cap = v;
+        }
         ETRY(hipMalloc(&e->d_draws, (size_t)e->draws_cap * 12 * (size_t)R));
     }
     ETRY(hipStreamSynchronize(e->stream));
@@ -479,6 +500,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.L = e->L;
     a.N = e->N;
     a.R = e->R;
+    a.lg = ilog2(e->L);
     a.rpw = e->rpw;
     a.stride = e->stride;
     a.a_begin = e->rng.attempts;
@@ -490,9 +512,10 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.fate = e->d_fate;
     a.counts = e->d_counts;
     a.check = check;
+    a.freeze = e->freeze;
     a.dn_thr = dn;
     a.up_thr = up;
-    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S);
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, is_pow2(e->L));
     if (check != CHECK_NONE) HIP_TRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0));
@@ -509,38 +532,51 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
 }
 
 // Advance all replicas by n attempts; `check` applies to the state at the end.
+// Launches carry at most 2^30 attempts per replica (32-bit launch-local
+// counters); MT19937 draws are generated in batches that fit the buffer.
 static int engine_advance(gasp_engine *e, uint64_t n, int check, double dn, double up) {
-    if (e->rng.kind != KIND_MT || n == 0) return launch_sweep(e, n, nullptr, check, dn, up);
+    if (n == 0) return launch_sweep(e, 0, nullptr, check, dn, up);
+    const uint64_t cap = e->rng.kind == KIND_MT ? std::min<uint64_t>(e->draws_cap, 1ull << 30) : (1ull << 30);
     uint64_t left = n;
     while (left > 0) {
-        const uint64_t nb = std::min(left, e->draws_cap);
-        uint32_t *st = (uint32_t *)((char *)e->d_rng + kRngMtOffset);
-        hipLaunchKernelGGL(mt_draws_kernel, dim3(e->R), dim3(64), 0, e->stream, st, e->R,
-                           3ull * e->rng.attempts, 3ull * nb, e->d_draws);
-        HIP_TRY(hipGetLastError());
-        const int rc = launch_sweep(e, nb, e->d_draws, left == nb ? check : CHECK_NONE, dn, up);
+        const uint64_t nb = std::min(left, cap);
+        if (e->rng.kind == KIND_MT) {
+            uint32_t *st = (uint32_t *)((char *)e->d_rng + kRngMtOffset);
+            hipLaunchKernelGGL(mt_draws_kernel, dim3(e->R), dim3(64), 0, e->stream, st, e->R,
+                               3ull * e->rng.attempts, 3ull * nb, e->d_draws);
+            HIP_TRY(hipGetLastError());
+        }
+        const int rc = launch_sweep(e, nb, e->rng.kind == KIND_MT ? e->d_draws : nullptr,
+                                    left == nb ? check : CHECK_NONE, dn, up);
         if (rc) return rc;
         left -= nb;
     }
     return GASP_OK;
 }
 
-static int engine_set_grids_impl(gasp_engine *e, const int *grids, int on_device) {
-    const size_t bytes = (size_t)e->R * e->N * sizeof(int);
+// Packs int32 grids into the native layout.  `ninputs` grids are given;
+// replica r takes grid r / (R / ninputs) (ninputs == R: one grid per replica).
+static int engine_set_grids_impl(gasp_engine *e, const int *grids, int on_device, int ninputs) {
+    const size_t bytes = (size_t)ninputs * e->N * sizeof(int);
     const int *src = grids;
     if (!on_device) {
-        if (!e->d_tmp32) HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+        if (!e->d_tmp32 || e->tmp32_bytes < bytes) {
+            dfree(e->d_tmp32);
+            e->d_tmp32 = nullptr;
+            HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+            e->tmp32_bytes = bytes;
+        }
         HIP_TRY(hipMemcpyAsync(e->d_tmp32, grids, bytes, hipMemcpyHostToDevice, e->stream));
         src = e->d_tmp32;
     }
-    const int N = e->N, R = e->R;
+    const int N = e->N, R = e->R, per = R / ninputs;
     const dim3 grid(1024), blk(256);
     if (e->method == 2) {
-        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (uint64_t *)e->d_lat, R, N, e->stride);
+        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (uint64_t *)e->d_lat, R, N, e->stride, per);
     } else if (e->method == 1) {
-        hipLaunchKernelGGL(pack_bits, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride);
+        hipLaunchKernelGGL(pack_bits, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride, per);
     } else {
-        hipLaunchKernelGGL(pack_hbm, grid, blk, 0, e->stream, src, (unsigned char *)e->d_lat, R, N, e->stride);
+        hipLaunchKernelGGL(pack_hbm, grid, blk, 0, e->stream, src, (unsigned char *)e->d_lat, R, N, e->stride, per);
     }
     HIP_TRY(hipGetLastError());
     return GASP_OK;
@@ -566,7 +602,12 @@ static int engine_unpack(gasp_engine *e, T *dst) {
 static int engine_get_grids_impl(gasp_engine *e, int *grids, int on_device) {
     const size_t bytes = (size_t)e->R * e->N * sizeof(int);
     if (on_device) return engine_unpack<int>(e, grids);
-    if (!e->d_tmp32) HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+    if (!e->d_tmp32 || e->tmp32_bytes < bytes) {
+        dfree(e->d_tmp32);
+        e->d_tmp32 = nullptr;
+        HIP_TRY(hipMalloc(&e->d_tmp32, bytes));
+        e->tmp32_bytes = bytes;
+    }
     int rc = engine_unpack<int>(e, e->d_tmp32);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(grids, e->d_tmp32, bytes, hipMemcpyDeviceToHost, e->stream));
@@ -707,8 +748,23 @@ extern "C" int gasp_engine_destroy(gasp_engine *e) {
 extern "C" int gasp_engine_set_grids(gasp_engine *e, const int *grids, int on_device) {
     clear_error();
     if (!e || !grids) return set_error(GASP_ERR_ARG, "gasp_engine_set_grids: NULL argument");
-    int rc = engine_set_grids_impl(e, grids, on_device);
+    int rc = engine_set_grids_impl(e, grids, on_device, e->R);
     return rc ? rc : engine_sync(e);
+}
+
+extern "C" int gasp_engine_set_inputs(gasp_engine *e, const int *inputs, int ninputs) {
+    clear_error();
+    if (!e || !inputs) return set_error(GASP_ERR_ARG, "gasp_engine_set_inputs: NULL argument");
+    if (ninputs < 1 || e->R % ninputs != 0)
+        return set_error(GASP_ERR_ARG, "Error: ngrids must be a multiple of ninputs!");
+    int rc = engine_set_grids_impl(e, inputs, 0, ninputs);
+    return rc ? rc : engine_sync(e);
+}
+
+extern "C" int gasp_engine_set_freeze(gasp_engine *e, int freeze) {
+    if (!e) return set_error(GASP_ERR_ARG, "gasp_engine_set_freeze: NULL engine");
+    e->freeze = freeze ? 1 : 0;
+    return GASP_OK;
 }
 
 extern "C" int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device) {
@@ -911,7 +967,7 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
         for (int i = 0; i < result_size; ++i) result[i] = 1.0f;
     const int check_mode = itask == 0 ? CHECK_COUNT : CHECK_FATES;
     int counts[4] = {0, 0, 0, 0};
-    int rc = engine_set_grids_impl(e, grids.d_ising_grids, 1);
+    int rc = engine_set_grids_impl(e, grids.d_ising_grids, 1, R);
     if (!rc && itask == 1)
         rc = hipMemcpyAsync(e->d_fate, grid_fate, (size_t)R * sizeof(int), hipMemcpyHostToDevice, e->stream) ==
                      hipSuccess

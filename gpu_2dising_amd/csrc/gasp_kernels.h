@@ -46,10 +46,11 @@ constexpr int kLdsLattice = 384;
 struct SweepArgs {
     void *lattice;           // native layout (see engine)
     int L, N, R;
+    int lg;                  // log2(L) when L is a power of two
     int rpw;                 // replicas per workgroup (layouts 0/1)
     int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: words per group
     uint64_t a_begin;        // per-replica attempt index of the first attempt
-    uint64_t n_attempts;     // attempts per replica in this launch
+    uint64_t n_attempts;     // attempts per replica in this launch (< 2^31)
     uint64_t seed;           // Philox key
     const uint32_t *draws;   // RNG_DRAWS: [R][3*n_attempts], draw stream from a_begin
     const uint64_t *thr;     // 10 thresholds, j = 5*s01 + (number of up neighbours)
@@ -57,6 +58,7 @@ struct SweepArgs {
     int *fate;               // CHECK_FATES: fates per replica
     int *counts;             // out: [0] #(m>up), [1] #fate 0, [2] #fate 1
     int check;
+    int freeze;              // skip replicas whose fate is already decided
     double dn_thr, up_thr;
 };
 
@@ -77,34 +79,62 @@ __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32
     }
 }
 
-// Exact site maps.
-//  Philox: site = floor(w*N / 2^32) with row = floor(w*L / 2^32) and col from
-//  the fractional part, which is the same integer.
-//  MT19937: row = floor(L * (u+0.5)*2^-32) = ((2u+1)*L) >> 33 (src/mc_cpu.c:51).
-__device__ __forceinline__ void philox_site(uint32_t w, uint32_t L, int &row, int &col) {
-    const uint64_t t = (uint64_t)w * L;
-    row = (int)(t >> 32);
-    col = (int)(((uint64_t)(uint32_t)t * L) >> 32);
-}
-__device__ __forceinline__ int real3_floor(uint32_t u, uint32_t L) {
-    return (int)((((uint64_t)u << 1) + 1ull) * L >> 33);
-}
-
+// Exact site maps and periodic stencils.
+//  Philox: site = floor(w*N / 2^32), computed as row = floor(w*L / 2^32) and
+//  col = floor(frac*L) from the fractional part, which is the same integer;
+//  for L = 2^lg this is w >> (32 - 2 lg).
+//  MT19937: row = floor(L * (u+0.5)*2^-32) = ((2u+1)*L) >> 33 (src/mc_cpu.c:51),
+//  which for L = 2^lg is u >> (32 - lg).
 struct Stencil {
     int c, up, dn, lt, rt;
 };
-__device__ __forceinline__ Stencil make_stencil(int row, int col, int L, int N) {
-    Stencil s;
-    s.c = row * L + col;
-    s.up = (row == 0) ? s.c + (N - L) : s.c - L;
-    s.dn = (row == L - 1) ? s.c - (N - L) : s.c + L;
-    s.lt = (col == 0) ? s.c + (L - 1) : s.c - 1;
-    s.rt = (col == L - 1) ? s.c - (L - 1) : s.c + 1;
-    return s;
-}
 __device__ __forceinline__ bool in_stencil(int x, const Stencil &s) {
     return (x == s.c) | (x == s.up) | (x == s.dn) | (x == s.lt) | (x == s.rt);
 }
+
+template <bool POW2>
+struct Geometry {
+    int L, N, lg;  // lg = log2(L) when POW2
+
+    __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {
+        Stencil s;
+        s.c = row * L + col;
+        s.up = (row == 0) ? s.c + (N - L) : s.c - L;
+        s.dn = (row == L - 1) ? s.c - (N - L) : s.c + L;
+        s.lt = (col == 0) ? s.c + (L - 1) : s.c - 1;
+        s.rt = (col == L - 1) ? s.c - (L - 1) : s.c + 1;
+        return s;
+    }
+    __device__ __forceinline__ Stencil from_site_pow2(int c) const {
+        const int m = N - 1, cm = L - 1;
+        Stencil s;
+        s.c = c;
+        s.up = (c - L) & m;
+        s.dn = (c + L) & m;
+        s.lt = (c & ~cm) | ((c - 1) & cm);
+        s.rt = (c & ~cm) | ((c + 1) & cm);
+        return s;
+    }
+    __device__ __forceinline__ Stencil philox(uint32_t w) const {
+        if constexpr (POW2) {
+            return from_site_pow2((int)(w >> (32 - 2 * lg)));
+        } else {
+            const uint64_t t = (uint64_t)w * (uint32_t)L;
+            const int row = (int)(t >> 32);
+            const int col = (int)(((uint64_t)(uint32_t)t * (uint32_t)L) >> 32);
+            return from_rowcol(row, col);
+        }
+    }
+    __device__ __forceinline__ Stencil mt(uint32_t urow, uint32_t ucol) const {
+        if constexpr (POW2) {
+            return from_site_pow2((int)(((urow >> (32 - lg)) << lg) | (ucol >> (32 - lg))));
+        } else {
+            const int row = (int)((((uint64_t)urow << 1) + 1ull) * (uint32_t)L >> 33);
+            const int col = (int)((((uint64_t)ucol << 1) + 1ull) * (uint32_t)L >> 33);
+            return from_rowcol(row, col);
+        }
+    }
+};
 
 // ------------------------------------------------------------ lattice access
 // Layout 2 (multispin): uint64 word per site, bit b = replica (64*group + b).
@@ -150,13 +180,12 @@ struct Attempt {
 };
 
 template <int LAT>
-__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, int row, int col, int L,
-                                            int N, uint32_t ua, bool en,
+__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stencil &st, uint32_t ua, bool en,
                                             const uint64_t *thr_lds) {
     Attempt t;
-    t.st = make_stencil(row, col, L, N);
-    t.s01 = lat.spin(t.st.c);
-    const int nup = lat.spin(t.st.up) + lat.spin(t.st.dn) + lat.spin(t.st.lt) + lat.spin(t.st.rt);
+    t.st = st;
+    t.s01 = lat.spin(st.c);
+    const int nup = lat.spin(st.up) + lat.spin(st.dn) + lat.spin(st.lt) + lat.spin(st.rt);
     const uint64_t T = thr_lds[5 * t.s01 + nup];
     t.acc = en && ((uint64_t)ua < T);
     return t;
@@ -197,7 +226,7 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
     }
 }
 
-template <int LAT, int RNG, int S>
+template <int LAT, int RNG, int S, bool POW2>
 __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -221,7 +250,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         r0 = blockIdx.x * a.rpw;
     }
     const int r = r0 + rl;
-    const bool active = r < a.R;
+    // live: the replica exists; active: it makes attempts in this launch
+    // (replicas past R, and when freezing those with a decided fate, do not)
+    const bool live = r < a.R;
+    const bool active = live && !(a.freeze && a.fate[r] != -1);
 
     // ---------------------------------------------------------- prologue
 #pragma unroll
@@ -251,53 +283,56 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     } else if constexpr (LAT == LAT_BITS) {
         lat.base = lat_lds + (size_t)rl * a.stride * 4;
     } else {
-        lat.base = (unsigned char *)a.lattice + (size_t)(active ? r : 0) * a.stride;
+        lat.base = (unsigned char *)a.lattice + (size_t)(live ? r : 0) * a.stride;
     }
 
-    const uint64_t a_end = a.a_begin + a.n_attempts;
-    uint64_t a0 = active ? a.a_begin : a_end;
+    // Launch-local attempt coordinates shifted by b = a_begin & 1 so that
+    // attempt pairs (2q, 2q+1) coincide with global Philox pairs P0 + q.
+    const uint32_t b = (uint32_t)(a.a_begin & 1ull);
+    const uint64_t P0 = a.a_begin >> 1;
+    const uint32_t e = (uint32_t)a.n_attempts + b;
+    uint32_t l0 = active ? b : e;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const Geometry<POW2> geo{L, N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS)
-        drw = a.draws + (size_t)(active ? r : 0) * (3 * a.n_attempts);
+        drw = a.draws + (size_t)(live ? r : 0) * (3 * a.n_attempts);
 
     // ---------------------------------------------------------- rounds
-    while (__ballot(a0 < a_end) != 0ull) {
-        const uint64_t p0 = a0 >> 1;
-        const uint64_t p = p0 + (uint64_t)slot;
-        const uint64_t attA = 2 * p, attB = attA + 1;
-        const bool enA = (attA >= a0) && (attA < a_end);
-        const bool enB = (attB >= a0) && (attB < a_end);
+    while (__ballot(l0 < e) != 0ull) {
+        const uint32_t q = (l0 >> 1) + (uint32_t)slot;
+        const uint32_t lA = 2 * q, lB = lA + 1;
+        const bool enA = (lA >= l0) && (lA < e);
+        const bool enB = (lB >= l0) && (lB < e);
 
-        int rowA, colA, rowB, colB;
+        Stencil sA, sB;
         uint32_t uaA, uaB;
         if constexpr (RNG == RNG_PHILOX) {
-            uint32_t c0 = (uint32_t)p, c1 = (uint32_t)(p >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
+            const uint64_t pair = P0 + q;
+            uint32_t c0 = (uint32_t)pair, c1 = (uint32_t)(pair >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
             philox4x32_10(c0, c1, c2, c3, k0, k1);
-            philox_site(c0, (uint32_t)L, rowA, colA);
+            sA = geo.philox(c0);
             uaA = c1;
-            philox_site(c2, (uint32_t)L, rowB, colB);
+            sB = geo.philox(c2);
             uaB = c3;
         } else {
             uint32_t u0 = 0, u1 = 0, u2 = 0, v0 = 0, v1 = 0, v2 = 0;
             if (enA) {
-                const uint32_t *d = drw + 3 * (attA - a.a_begin);
+                const uint32_t *d = drw + 3 * (size_t)(lA - b);
                 u0 = d[0]; u1 = d[1]; u2 = d[2];
             }
             if (enB) {
-                const uint32_t *d = drw + 3 * (attB - a.a_begin);
+                const uint32_t *d = drw + 3 * (size_t)(lB - b);
                 v0 = d[0]; v1 = d[1]; v2 = d[2];
             }
-            rowA = real3_floor(u0, (uint32_t)L);
-            colA = real3_floor(u1, (uint32_t)L);
+            sA = geo.mt(u0, u1);
             uaA = u2;
-            rowB = real3_floor(v0, (uint32_t)L);
-            colB = real3_floor(v1, (uint32_t)L);
+            sB = geo.mt(v0, v1);
             uaB = v2;
         }
 
-        const Attempt A = evaluate<LAT>(lat, rowA, colA, L, N, uaA, enA, thr_lds);
-        const Attempt B = evaluate<LAT>(lat, rowB, colB, L, N, uaB, enB, thr_lds);
+        const Attempt A = evaluate<LAT>(lat, sA, uaA, enA, thr_lds);
+        const Attempt B = evaluate<LAT>(lat, sB, uaB, enB, thr_lds);
 
         bool conflA = false;
         bool conflB = A.acc && in_stencil(A.st.c, B.st);
@@ -337,12 +372,13 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             const uint64_t iA = __ballot(invA), iB = __ballot(invB);
             const uint64_t gm = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << gbase);
             const uint64_t x = (iA | iB) & gm;
+            const uint32_t q0 = l0 >> 1;
             if (x == 0ull) {
-                const uint64_t nxt = 2 * (p0 + S);
-                a0 = nxt < a_end ? nxt : a_end;
+                const uint32_t nxt = 2 * (q0 + S);
+                l0 = nxt < e ? nxt : e;
             } else {
                 const int f = __builtin_ctzll(x);
-                a0 = 2 * (p0 + (uint64_t)(f - gbase)) + (((iA >> f) & 1ull) ? 0ull : 1ull);
+                l0 = 2 * (q0 + (uint32_t)(f - gbase)) + (((iA >> f) & 1ull) ? 0u : 1u);
             }
             if constexpr (LAT == LAT_HBM) {
                 // lanes of a replica must see each other's byte stores next round
@@ -354,10 +390,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         } else {
             (void)invA;
             if (invB) {
-                a0 = attB;
+                l0 = lB;
             } else {
-                const uint64_t nxt = 2 * p0 + 2;
-                a0 = nxt < a_end ? nxt : a_end;
+                const uint32_t nxt = lA + 2;
+                l0 = nxt < e ? nxt : e;
             }
         }
     }
@@ -386,7 +422,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
         unsigned c = 0;
-        if (active) {
+        if (live) {
             if constexpr (LAT == LAT_BITS) {
                 const unsigned *w = (const unsigned *)lat.base;
                 for (int i = slot; i < a.stride; i += S) c += __popc(w[i]);
@@ -399,7 +435,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         }
 #pragma unroll
         for (int off = S / 2; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
-        finish_replica(a, active && slot == 0, r, c);
+        finish_replica(a, live && slot == 0, r, c);
         if constexpr (LAT == LAT_BITS) {
             const int nrep = min(a.rpw, a.R - r0);
             const int ndw = nrep * a.stride;
@@ -413,7 +449,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 // ------------------------------------------------------------ layout kernels
 // int32 +-1 grids [R][N] <-> native layouts.  1-D grid-stride loops over
 // (replica or group, element) so that any R fits the launch limits.
-__global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict__ dst, int R, int N, int stride) {
+// `per` consecutive replicas share one source grid (1: one grid per replica;
+// R / ninputs: committor inputs replicated on the device).
+__global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict__ dst, int R, int N, int stride,
+                               int per) {
     const int G = (R + 63) / 64;
     const size_t total = (size_t)G * N;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
@@ -421,7 +460,7 @@ __global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict
         uint64_t w = 0;
         const int nb = min(64, R - 64 * g);
         for (int b = 0; b < nb; ++b)
-            if (src[(size_t)(64 * g + b) * N + site] == 1) w |= 1ull << b;
+            if (src[(size_t)((64 * g + b) / per) * N + site] == 1) w |= 1ull << b;
         dst[(size_t)g * stride + site] = w;
     }
 }
@@ -438,14 +477,15 @@ __global__ void unpack_multispin(const uint64_t *__restrict__ src, T *__restrict
     }
 }
 
-__global__ void pack_bits(const int *__restrict__ src, unsigned *__restrict__ dst, int R, int N, int stride) {
+__global__ void pack_bits(const int *__restrict__ src, unsigned *__restrict__ dst, int R, int N, int stride,
+                          int per) {
     const size_t total = (size_t)R * stride;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
         const int r = (int)(i / stride), w = (int)(i - (size_t)r * stride);
         unsigned v = 0;
         for (int k = 0; k < 32; ++k) {
             const int site = 32 * w + k;
-            if (site < N && src[(size_t)r * N + site] == 1) v |= 1u << k;
+            if (site < N && src[(size_t)(r / per) * N + site] == 1) v |= 1u << k;
         }
         dst[i] = v;
     }
@@ -461,11 +501,12 @@ __global__ void unpack_bits(const unsigned *__restrict__ src, T *__restrict__ ds
     }
 }
 
-__global__ void pack_hbm(const int *__restrict__ src, unsigned char *__restrict__ dst, int R, int N, int stride) {
+__global__ void pack_hbm(const int *__restrict__ src, unsigned char *__restrict__ dst, int R, int N, int stride,
+                         int per) {
     const size_t total = (size_t)R * stride;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
         const int r = (int)(i / stride), site = (int)(i - (size_t)r * stride);
-        dst[i] = (site < N && src[(size_t)r * N + site] == 1) ? 1 : 0;
+        dst[i] = (site < N && src[(size_t)(r / per) * N + site] == 1) ? 1 : 0;
     }
 }
 

@@ -200,6 +200,14 @@ int gasp_engine_create(int L, int ngrids, int gpu_method, gasp_rng_state *d_stat
 int gasp_engine_destroy(gasp_engine *e);
 /* int32 +-1 spins, ngrids*L*L; `on_device` selects device or host memory. */
 int gasp_engine_set_grids(gasp_engine *e, const int *grids, int on_device);
+/* Host int32 input grids [ninputs][L*L]; replica r starts from input
+ * r / (ngrids / ninputs), as init_grids_from_array (src/grid.c:41-71) but
+ * replicated on the device. */
+int gasp_engine_set_inputs(gasp_engine *e, const int *inputs, int ninputs);
+/* 1: replicas whose fate is decided stop sweeping (their state no longer
+ * changes); pB and fates are unaffected.  Default 0 (the reference keeps
+ * sweeping them). */
+int gasp_engine_set_freeze(gasp_engine *e, int freeze);
 int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device);
 /* Advance every replica by nsweeps sweeps.  check: 0 none, 1 count replicas
  * with m > up_thr, 2 update fates (sticky; 1 if m > up_thr, 0 if

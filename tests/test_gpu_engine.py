@@ -101,3 +101,40 @@ def test_largest_cluster(gpu, engine_lib, oracle, L):
     ref_dn = np.array([O.lib().orc_largest_cluster(L, g0[r].reshape(-1), -1) for r in range(R)])
     assert np.array_equal(got_up, ref_up)
     assert np.array_equal(got_dn, ref_dn)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method):
+    """gasp_engine_set_inputs replicates input grids exactly as uploading the
+    replicated grids; freezing decided replicas changes no fate at any check,
+    and no undecided replica's trajectory."""
+    G = engine_lib
+    L, nin, per = 16, 4, 24
+    R = nin * per
+    inputs = np.stack([np.where(np.add.outer(np.arange(L), np.arange(L)) % (k + 3) == 0, 1, -1)
+                       for k in range(nin)]).astype(np.int32)
+    print(f"[freeze] method={method}", flush=True)
+    runs = []
+    for freeze in (False, True):
+        rng = G.RngState(R, 99)
+        with G.Engine(L, R, method, rng, 0.54, 0.07) as e:
+            if freeze:
+                e.set_inputs(inputs)
+            else:
+                e.set_grids(np.repeat(inputs, per, axis=0))
+            assert np.array_equal(e.get_grids(), np.repeat(inputs, per, axis=0))
+            e.set_freeze(freeze)
+            e.set_fates(np.full(R, -1, dtype=np.int32))
+            e.sweep(0, G.CHECK_FATES, -0.8, 0.6)
+            hist = []
+            for _ in range(30):
+                e.sweep(5, G.CHECK_FATES, -0.8, 0.6)
+                f = e.fates()
+                hist.append((e.counts().copy(), f, e.get_grids()))
+        rng.free()
+        runs.append(hist)
+    for (c0, f0, g0), (c1, f1, g1) in zip(*runs):
+        assert np.array_equal(c0[1:], c1[1:]) and np.array_equal(f0, f1)
+        und = f1 == -1
+        assert np.array_equal(g0[und], g1[und])
+    assert (runs[1][-1][1] != -1).sum() > 0  # some replicas did freeze

@@ -1,0 +1,79 @@
+"""Statistical parity with the reference's published physics (its notebooks are
+its only results; the GPU runs there used cuRAND streams, so agreement is
+statistical, within the stated standard errors):
+
+  - pB of the 13x13 square droplet at L=64, beta=0.54, h=0.07, thresholds
+    dn=-0.93 / up=0.90: 10-repeat mean 0.78184 +- 0.00597 (SE)
+    (Jupyter/GASP_Tests.ipynb:839, cells 21-24);
+  - the metastable magnetisation mode -0.94 (Jupyter/GASP_Tests.ipynb:273);
+  - the nucleation rate per grid k = 1.5032e-4 per sweep from a fit of
+    1 - exp(-k (t - t0)) at L=64 (Jupyter/GASP_rates.ipynb:178).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BETA, H = 0.54, 0.07
+
+
+def _gasp(engine_lib):
+    if engine_lib.PKG_DIR not in sys.path:
+        sys.path.insert(0, engine_lib.PKG_DIR)
+    import gasp
+
+    return gasp
+
+
+def _square(L=64, size=6):
+    # Jupyter/GASP_Tests.ipynb cell 21: +1 where |row - L/2| <= 6 and |col - L/2| <= 6
+    g = np.ones((L, L), dtype=np.int8)
+    r = np.arange(L)
+    out = (np.abs(r[:, None] - L / 2) > size) | (np.abs(r[None, :] - L / 2) > size)
+    g[out] = -1
+    return g
+
+
+def test_square_droplet_committor(gpu, engine_lib, tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    gasp = _gasp(engine_lib)
+    R = 8192
+    print(f"[physics] square droplet pB with {R} replicas", flush=True)
+    (pB, err), = gasp.run_committor_calc(64, R, 50000, BETA, H, grid_output_int=100000, grid_input="NumPy",
+                                         grid_array=[_square()], nsms=1, keep_grids=False, seed=11)
+    ref, ref_se = 0.78184, 0.00597
+    se = np.sqrt(pB * (1 - pB) / R)
+    assert abs(err - se) < 0.5 * se  # bootstrap SD ~ binomial SE
+    assert abs(pB - ref) < 3.5 * np.hypot(se, ref_se), (pB, err)
+
+
+def test_metastable_mode_and_nucleation_rate(gpu, engine_lib):
+    G = engine_lib
+    L, R, nchk, spc = 64, 16384, 200, 100
+    print(f"[physics] swarm L={L} R={R} {nchk * spc} sweeps", flush=True)
+    rng = G.RngState(R, 31337)
+    frac, hist = [], np.zeros(50)
+    edges = np.linspace(-1.0, -0.8, 51)
+    with G.Engine(L, R, 2, rng, BETA, H) as e:
+        e.set_grids(-np.ones((R, L, L), dtype=np.int32))
+        for _ in range(nchk):
+            e.sweep(spc, G.CHECK_COUNT, -0.9, 0.9)
+            frac.append(e.counts()[0] / R)
+            m = e.magnetisation()
+            hist += np.histogram(m, bins=edges)[0]
+    rng.free()
+    # metastable mode: the reference histogram (range (-1, -0.8), 50 bins) peaks at -0.94
+    mode = edges[np.argmax(hist)]
+    assert abs(mode - (-0.94)) <= 0.0041 + 1e-12, mode
+    # nucleation rate
+    from scipy.optimize import curve_fit
+
+    t = np.arange(1, nchk + 1) * spc
+    f = np.array(frac)
+    k, t0 = curve_fit(lambda t, k, t0: 1 - np.exp(-k * (t - t0)), t[f > 0], f[f > 0], p0=(1.5e-4, 500))[0]
+    print(f"k = {k:.4e} per sweep, t0 = {t0:.0f}", flush=True)
+    # the reference's k comes from 1024 replicas (about 1000 events): ~4% standard error
+    assert abs(k - 1.5032e-4) < 0.15 * 1.5032e-4, k
