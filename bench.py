@@ -58,6 +58,59 @@ def committor_inputs(n: int, L: int = 64, seed: int = 12345) -> np.ndarray:
     return out
 
 
+def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 20000,
+                  chk: int = 100, time_cap: float = 240.0) -> dict:
+    """Committor shooting (itask 1): every input grid gets `per` replicas with
+    independent Philox streams, run until all fates are decided against
+    dn = -0.90 / up = +0.90 (the CLI thresholds) or max_sweeps; replicas with a
+    decided fate stop sweeping.  pB per input and its bootstrap error
+    (bootstrap_errbar, libc rand(), as the reference) are computed on the host."""
+    import ctypes as C
+
+    inputs = committor_inputs(n_inputs, L)
+    R_c = n_inputs * per
+    rng = G.RngState(R_c, 777)
+    e = G.Engine(L, R_c, 2, rng, BETA, H)
+    e.set_inputs(inputs)
+    e.set_freeze(True)
+    e.set_fates(np.full(R_c, -1, dtype=np.int32))
+    e.sync()
+    t0 = time.perf_counter()
+    e.sweep(0, G.CHECK_FATES, DN_THR, UP_THR)
+    sweeps, c = 0, e.counts()
+    while sweeps < max_sweeps and c[1] + c[2] < R_c and time.perf_counter() - t0 < time_cap:
+        e.sweep(chk, G.CHECK_FATES, DN_THR, UP_THR)
+        sweeps += chk
+        c = e.counts()
+    fates = e.fates()
+    t_gpu = time.perf_counter() - t0
+    kern_ms = e.kernel_ms()
+    e.close()
+    rng.free()
+    f = fates.reshape(n_inputs, per)
+    nB = (f == 1).sum(1)
+    nres = (f >= 0).sum(1)
+    pB = np.where(nres > 0, nB / np.maximum(nres, 1), np.nan).astype(np.float32)
+    t1 = time.perf_counter()
+    lib = G.lib()
+    errs = [lib.bootstrap_errbar(per, per, np.ascontiguousarray(f[i]).ctypes.data_as(C.POINTER(C.c_int)))
+            for i in range(n_inputs)]
+    t_boot = time.perf_counter() - t1
+    return {
+        "metric": "committor pB estimates/sec (shooting)",
+        "value": n_inputs / (t_gpu + t_boot),
+        "unit": "pB estimates/s",
+        "gpu_only_value": n_inputs / t_gpu,
+        "inputs": n_inputs, "replicas_per_input": per, "sweeps": sweeps,
+        "unresolved": int(R_c - c[1] - c[2]), "gpu_seconds": t_gpu, "kernel_seconds": kern_ms / 1e3,
+        "bootstrap_seconds": t_boot,
+        "pB_quantiles": [float(x) for x in np.nanquantile(pB, [0.0, 0.25, 0.5, 0.75, 1.0])],
+        "mean_bootstrap_error": float(np.mean(errs)),
+        "workload": "512 grids: -1 sea, +1 disc of radius U[4,14] at a random centre, 5% of spins "
+                    "flipped; 1024 replicas each; thresholds -0.90/+0.90; <= 20000 sweeps",
+    }
+
+
 def fit_rate(sweeps: np.ndarray, frac: np.ndarray):
     """Least-squares fit of 1 - exp(-k (t - t0)) (the reference notebooks' model)."""
     try:
@@ -108,6 +161,7 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=0, help="slots per replica (0 = tuner)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the grid dumps")
+    ap.add_argument("--no-committor", action="store_true", help="skip the committor shooting leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas are independent "
@@ -199,6 +253,8 @@ def main() -> int:
                 out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
         except Exception:
             pass
+    if not args.no_committor:
+        out["committor"] = committor_leg(G)
     if not args.no_cpu:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb

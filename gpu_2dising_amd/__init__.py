@@ -33,6 +33,9 @@ EXPORTS = (
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
+    "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
+    "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
+    "gasp_cnn_set_data", "gasp_cnn_train_epoch",
 )
 
 CHECK_NONE, CHECK_COUNT, CHECK_FATES = 0, 1, 2
@@ -144,6 +147,20 @@ def lib() -> C.CDLL:
     L.gasp_engine_kernel_ms.restype = C.c_double
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
+    i8p = C.POINTER(C.c_int8)
+    for name in ("gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
+                 "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
+                 "gasp_cnn_set_data", "gasp_cnn_train_epoch"):
+        getattr(L, name).restype = C.c_int
+    L.gasp_cnn_create.argtypes = [C.c_int, C.c_int, C.POINTER(vp)]
+    L.gasp_cnn_destroy.argtypes = [vp]
+    L.gasp_cnn_set_params.argtypes = [vp, fp]
+    L.gasp_cnn_get_params.argtypes = [vp, fp]
+    L.gasp_cnn_predict.argtypes = [vp, i8p, C.c_int, fp]
+    L.gasp_cnn_gradient.argtypes = [vp, i8p, fp, C.c_int, fp, fp]
+    L.gasp_cnn_train_step.argtypes = [vp, i8p, fp, C.c_int, C.c_float, fp]
+    L.gasp_cnn_set_data.argtypes = [vp, i8p, fp, C.c_int]
+    L.gasp_cnn_train_epoch.argtypes = [vp, ip, C.c_int, C.c_int, C.c_float, fp]
     _lib = L
     return L
 
@@ -250,6 +267,75 @@ class Engine:
     def close(self) -> None:
         if self.ptr:
             check(lib().gasp_engine_destroy(self.ptr))
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Surrogate:
+    """Committor-surrogate CNN over the C ABI (gasp_cnn_*)."""
+
+    def __init__(self, L: int = 64, max_batch: int = 256):
+        self.L, self.max_batch = L, max_batch
+        self.ptr = C.c_void_p()
+        check(lib().gasp_cnn_create(L, max_batch, C.byref(self.ptr)))
+        self.nparam = lib().gasp_cnn_num_params()
+
+    @staticmethod
+    def _spins(spins):
+        return np.ascontiguousarray(spins, dtype=np.int8)
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.nparam
+        check(lib().gasp_cnn_set_params(self.ptr, _fp(p)))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, dtype=np.float32)
+        check(lib().gasp_cnn_get_params(self.ptr, _fp(out)))
+        return out
+
+    def predict(self, spins: np.ndarray) -> np.ndarray:
+        s = self._spins(spins)
+        out = np.empty(s.shape[0], dtype=np.float32)
+        check(lib().gasp_cnn_predict(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int8)), s.shape[0], _fp(out)))
+        return out
+
+    def gradient(self, spins: np.ndarray, labels: np.ndarray):
+        s = self._spins(spins)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = C.c_float()
+        check(lib().gasp_cnn_gradient(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int8)), _fp(y), s.shape[0],
+                                      _fp(g), C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, spins: np.ndarray, labels: np.ndarray, lr: float) -> float:
+        s = self._spins(spins)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        loss = C.c_float()
+        check(lib().gasp_cnn_train_step(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int8)), _fp(y), s.shape[0],
+                                        lr, C.byref(loss)))
+        return loss.value
+
+    def set_data(self, spins: np.ndarray, labels: np.ndarray) -> None:
+        s = self._spins(spins)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        check(lib().gasp_cnn_set_data(self.ptr, s.ctypes.data_as(C.POINTER(C.c_int8)), _fp(y), s.shape[0]))
+
+    def train_epoch(self, order: np.ndarray, batch: int, lr: float) -> float:
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        loss = C.c_float()
+        check(lib().gasp_cnn_train_epoch(self.ptr, _ip(o), o.size, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().gasp_cnn_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __enter__(self):

@@ -48,11 +48,13 @@ def _run(cmd: list[str]) -> None:
 def build_lib() -> str:
     os.makedirs(BUILD, exist_ok=True)
     eng_o = os.path.join(BUILD, "gasp_engine.o")
+    cnn_o = os.path.join(BUILD, "gasp_cnn.o")
     host_o = os.path.join(BUILD, "gasp_host.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gasp_engine.hip"), "-o", eng_o])
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gasp_cnn.hip"), "-o", cnn_o])
     _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "gasp_host.cpp"), "-o", host_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, host_o])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, cnn_o, host_o])
     os.replace(tmp, LIB)
     return LIB
 

@@ -227,6 +227,27 @@ double gasp_engine_kernel_ms(gasp_engine *e);
 /* The engine's HIP stream (hipStream_t). */
 void *gasp_engine_stream(gasp_engine *e);
 
+/* ------------------------------------------------ committor surrogate (CNN) */
+/* No counterpart in the reference: the learned addition.  A 4-layer circular
+ * 3x3 CNN (32 channels) + global average pool + MLP (64) + sigmoid mapping an
+ * L x L spin grid (L a power of two >= 16) to pB, trained with binary
+ * cross-entropy on shooting labels.  BF16 MFMA kernels with FP32
+ * accumulation, FP32 master weights, Adam.  Spins are int8 +-1 [n][L][L];
+ * parameters a float32 vector of gasp_cnn_num_params() entries. */
+typedef struct gasp_cnn gasp_cnn;
+int gasp_cnn_num_params(void);
+int gasp_cnn_create(int L, int max_batch, gasp_cnn **out);
+int gasp_cnn_destroy(gasp_cnn *c);
+int gasp_cnn_set_params(gasp_cnn *c, const float *host_params); /* resets Adam */
+int gasp_cnn_get_params(gasp_cnn *c, float *host_params);
+int gasp_cnn_predict(gasp_cnn *c, const int8_t *spins, int n, float *pB);
+/* gradient of the mean loss over n <= max_batch grids (no update) */
+int gasp_cnn_gradient(gasp_cnn *c, const int8_t *spins, const float *labels, int n, float *grad, float *loss);
+int gasp_cnn_train_step(gasp_cnn *c, const int8_t *spins, const float *labels, int n, float lr, float *loss);
+/* device-resident training set, then epochs over it in a caller-given order */
+int gasp_cnn_set_data(gasp_cnn *c, const int8_t *spins, const float *labels, int count);
+int gasp_cnn_train_epoch(gasp_cnn *c, const int *order, int count, int batch, float lr, float *mean_loss);
+
 #ifdef __cplusplus
 }
 #endif
