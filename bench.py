@@ -111,6 +111,71 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
     }
 
 
+BATCH_GROUPS = [(32, 8192, 2), (48, 4096, 2), (64, 4096, 2), (96, 2048, 2), (256, 256, 0)]
+
+
+def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
+    """Uneven batch in one captured graph: L in {32, 48, 64, 96} (multispin,
+    LDS) with {8192, 4096, 4096, 2048} replicas and L = 256 (HBM path) with
+    256; beta ~ U[0.50, 0.60], h ~ U[0.03, 0.10] per group; itask 0 (count
+    nucleated) and itask 1 (fates from droplet inputs) alternate.  Launch
+    shapes come from the autotuner.  One replay = `chk` sweeps of every group
+    plus its threshold check."""
+    rs = np.random.default_rng(seed)
+    tuned = []
+    for (Lg, Rg, m) in BATCH_GROUPS:
+        s, w, rate = G.autotune(m, Lg, Rg, 2)
+        tuned.append({"L": Lg, "replicas": Rg, "method": m, "slots": s, "waves": w, "probe_rate": rate})
+    engines, rngs, checks, dn, up = [], [], [], [], []
+    for k, (Lg, Rg, m) in enumerate(BATCH_GROUPS):
+        beta, h = rs.uniform(0.50, 0.60), rs.uniform(0.03, 0.10)
+        tuned[k].update(beta=beta, h=h, itask=k % 2)
+        rng = G.RngState(Rg, seed + k)
+        e = G.Engine(Lg, Rg, m, rng, beta, h)
+        if k % 2 == 0:
+            e.set_grids(-np.ones((Rg, Lg, Lg), dtype=np.int32))
+            checks.append(G.CHECK_COUNT)
+        else:
+            r = np.arange(Lg)
+            ins = []
+            for j in range(16):  # droplets of radius L/10 .. L/5
+                rad = Lg / 10 + (Lg / 10) * j / 15
+                d2 = (r[:, None] - Lg // 2) ** 2 + (r[None, :] - Lg // 2) ** 2
+                ins.append(np.where(d2 <= rad * rad, 1, -1))
+            e.set_inputs(np.stack(ins).astype(np.int32))
+            e.set_fates(np.full(Rg, -1, dtype=np.int32))
+            checks.append(G.CHECK_FATES)
+        dn.append(-0.9)
+        up.append(0.9)
+        engines.append(e)
+        rngs.append(rng)
+    attempts = sum(Lg * Lg * Rg for (Lg, Rg, _) in BATCH_GROUPS) * chk
+    graph = G.Graph(engines, [chk] * len(engines), checks, dn, up)
+    graph.launch(1)  # warm-up
+    ms0 = graph.ms()
+    t0 = time.perf_counter()
+    graph.launch(replays)
+    wall = time.perf_counter() - t0
+    g_ms = (graph.ms() - ms0) / replays
+    graph.close()
+    # the same work one group after another on the engines' own streams
+    t0 = time.perf_counter()
+    for _ in range(replays):
+        for e, c, d, u in zip(engines, checks, dn, up):
+            e.sweep(chk, c, d, u)
+            e.sync()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / replays
+    for e in engines:
+        e.close()
+    for r in rngs:
+        r.free()
+    return {"metric": "batched spin-flip attempts/sec (one captured graph, uneven L)",
+            "value": attempts / (g_ms * 1e-3), "unit": "attempts/s",
+            "wall_value": attempts * replays / wall, "graph_ms_per_replay": g_ms,
+            "sequential_ms_per_replay": seq_ms, "sweeps_per_replay": chk, "replays": replays,
+            "attempts_per_replay": attempts, "groups": tuned}
+
+
 def fit_rate(sweeps: np.ndarray, frac: np.ndarray):
     """Least-squares fit of 1 - exp(-k (t - t0)) (the reference notebooks' model)."""
     try:
@@ -162,6 +227,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the grid dumps")
     ap.add_argument("--no-committor", action="store_true", help="skip the committor shooting leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas are independent "
@@ -253,8 +319,11 @@ def main() -> int:
                 out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
         except Exception:
             pass
+    G.set_tuning(0, 0)
     if not args.no_committor:
         out["committor"] = committor_leg(G)
+    if not args.no_batched:
+        out["batched"] = batched_leg(G)
     if not args.no_cpu:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb

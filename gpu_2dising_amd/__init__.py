@@ -33,6 +33,7 @@ EXPORTS = (
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
+    "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
     "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
     "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
     "gasp_cnn_set_data", "gasp_cnn_train_epoch",
@@ -147,6 +148,17 @@ def lib() -> C.CDLL:
     L.gasp_engine_kernel_ms.restype = C.c_double
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
+    dp = C.POINTER(C.c_double)
+    L.gasp_graph_create.argtypes = [C.POINTER(vp), C.c_int, ip, ip, ip, dp, dp, C.POINTER(vp)]
+    L.gasp_graph_create.restype = C.c_int
+    L.gasp_graph_launch.argtypes = [vp, C.c_int]
+    L.gasp_graph_launch.restype = C.c_int
+    L.gasp_graph_ms.argtypes = [vp]
+    L.gasp_graph_ms.restype = C.c_double
+    L.gasp_graph_destroy.argtypes = [vp]
+    L.gasp_graph_destroy.restype = C.c_int
+    L.gasp_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp]
+    L.gasp_autotune.restype = C.c_int
     i8p = C.POINTER(C.c_int8)
     for name in ("gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
                  "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
@@ -343,6 +355,41 @@ class Surrogate:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Graph:
+    """One captured hipGraph advancing several engines concurrently."""
+
+    def __init__(self, engines, nsweeps, checks, dn, up, launches=None):
+        n = len(engines)
+        arr = (C.c_void_p * n)(*[e.ptr.value for e in engines])
+        ns = np.ascontiguousarray(nsweeps, dtype=np.int32)
+        ck = np.ascontiguousarray(checks, dtype=np.int32)
+        lc = None if launches is None else np.ascontiguousarray(launches, dtype=np.int32)
+        d = np.ascontiguousarray(dn, dtype=np.float64)
+        u = np.ascontiguousarray(up, dtype=np.float64)
+        self.engines = engines
+        self.ptr = C.c_void_p()
+        check(lib().gasp_graph_create(arr, n, _ip(ns), None if lc is None else _ip(lc), _ip(ck),
+                                      d.ctypes.data_as(C.POINTER(C.c_double)),
+                                      u.ctypes.data_as(C.POINTER(C.c_double)), C.byref(self.ptr)))
+
+    def launch(self, replays: int = 1) -> None:
+        check(lib().gasp_graph_launch(self.ptr, replays))
+
+    def ms(self) -> float:
+        return lib().gasp_graph_ms(self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            lib().gasp_graph_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def autotune(method: int, L: int, ngrids: int, probe_sweeps: int = 5):
+    s, w, r = C.c_int(), C.c_int(), C.c_double()
+    check(lib().gasp_autotune(method, L, ngrids, probe_sweeps, C.byref(s), C.byref(w), C.byref(r)))
+    return s.value, w.value, r.value
 
 
 def set_tuning(slots: int = 0, waves: int = 0) -> None:

@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -243,7 +245,19 @@ static size_t lds_bytes_for(int method, int N, int S, int waves) {
 // Chooses slots per replica S and waves per workgroup.  Aim: about four
 // resident waves per SIMD (16 per CU) over the whole device, within the LDS
 // budget; explicit settings from gasp_set_tuning win when they are valid.
+// Shapes picked by gasp_autotune, keyed by (method, L, ngrids); used when no
+// explicit gasp_set_tuning override is active.
+static std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;
+
 static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
+    if (g_tune_slots <= 0 && g_tune_waves <= 0) {
+        auto it = g_tuned.find(std::make_tuple(method, L, R));
+        if (it != g_tuned.end()) {
+            *S_out = it->second.first;
+            *W_out = it->second.second;
+            return;
+        }
+    }
     DeviceInfo d = device_info();
     const int ncu = d.ok ? d.ncu : 256;
     const size_t lds_block = d.ok ? d.lds_block : 65536;
@@ -494,7 +508,7 @@ static int engine_sync(gasp_engine *e) {
 }
 
 static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *draws, int check,
-                        double dn, double up) {
+                        double dn, double up, bool capture = false) {
     SweepArgs a{};
     a.lattice = e->d_lat;
     a.L = e->L;
@@ -504,6 +518,9 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.rpw = e->rpw;
     a.stride = e->stride;
     a.a_begin = e->rng.attempts;
+    // in a captured graph the attempt counter lives in the RNG header on the
+    // device and is advanced by a kernel after the sweep
+    a.a_begin_dev = capture ? (const uint64_t *)((char *)e->d_rng + offsetof(RngHeader, attempts)) : nullptr;
     a.n_attempts = n_attempts;
     a.seed = e->rng.seed;
     a.draws = draws;
@@ -516,7 +533,18 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.dn_thr = dn;
     a.up_thr = up;
     SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, is_pow2(e->L));
-    if (check != CHECK_NONE) HIP_TRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
+    if (check != CHECK_NONE) {
+        hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
+        HIP_TRY(hipGetLastError());
+    }
+    if (capture) {
+        hipLaunchKernelGGL(fn, dim3(e->nblocks), dim3(e->block), e->lds, e->stream, a);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(advance_counter, dim3(1), dim3(64), 0, e->stream,
+                           (uint64_t *)((char *)e->d_rng + offsetof(RngHeader, attempts)), n_attempts);
+        HIP_TRY(hipGetLastError());
+        return GASP_OK;
+    }
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
@@ -714,7 +742,7 @@ static int engine_cluster(gasp_engine *e, int spin) {
 }
 
 static int engine_check_cv(gasp_engine *e, const float *d_cv, int check, double dn, double up) {
-    HIP_TRY(hipMemsetAsync(e->d_counts, 0, 4 * sizeof(int), e->stream));
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
     hipLaunchKernelGGL(check_cv_kernel, dim3((e->R + 255) / 256), dim3(256), 0, e->stream, d_cv, e->R, check,
                        dn, up, e->d_fate, e->d_counts);
     HIP_TRY(hipGetLastError());
@@ -826,6 +854,195 @@ extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
 }
 
 extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+// ============================================================ batched graph
+// One captured hipGraph advancing several engines (different L, method,
+// temperature, task) concurrently: the capture stream forks to every engine's
+// stream; each branch clears its counters (if it checks), runs `launches`
+// sweep kernels of nsweeps/launches sweeps each and advances its attempt
+// counter on the device; the branches join back.  A replay is one check
+// interval of every engine.
+struct gasp_graph {
+    std::vector<gasp_engine *> eng;
+    std::vector<uint64_t> per_replay;  // attempts per replica per replay
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double ms = 0.0;
+    int replays = 0;
+};
+
+extern "C" int gasp_graph_destroy(gasp_graph *g) {
+    if (!g) return GASP_OK;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (hipEvent_t e : g->ev) (void)hipEventDestroy(e);
+    if (g->t0) (void)hipEventDestroy(g->t0);
+    if (g->t1) (void)hipEventDestroy(g->t1);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+    return GASP_OK;
+}
+
+extern "C" int gasp_graph_create(gasp_engine **engines, int n, const int *nsweeps, const int *launches,
+                                 const int *checks, const double *dn, const double *up, gasp_graph **out) {
+    clear_error();
+    if (!engines || n < 1 || !nsweeps || !checks || !dn || !up || !out)
+        return set_error(GASP_ERR_ARG, "gasp_graph_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!engines[i]) return set_error(GASP_ERR_ARG, "gasp_graph_create: NULL engine");
+        if (engines[i]->rng.kind != KIND_PHILOX)
+            return set_error(GASP_ERR_ARG, "gasp_graph_create: graphs need counter-based (Philox) streams");
+        const int k = launches ? launches[i] : 1;
+        if (nsweeps[i] < 1 || k < 1 || nsweeps[i] % k != 0)
+            return set_error(GASP_ERR_ARG, "gasp_graph_create: nsweeps must be a positive multiple of launches");
+        const uint64_t per = (uint64_t)(nsweeps[i] / k) * engines[i]->N;
+        if (per >= (1ull << 30)) return set_error(GASP_ERR_ARG, "gasp_graph_create: launch too long");
+    }
+    gasp_graph *g = new gasp_graph();
+#define GTRY(x)                                                                                 \
+    do {                                                                                        \
+        hipError_t q_ = (x);                                                                    \
+        if (q_ != hipSuccess) {                                                                 \
+            gasp_graph_destroy(g);                                                              \
+            return set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(q_), __FILE__, __LINE__); \
+        }                                                                                       \
+    } while (0)
+    GTRY(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    GTRY(hipEventCreate(&g->t0));
+    GTRY(hipEventCreate(&g->t1));
+    for (int i = 0; i < n; ++i) {
+        GTRY(engine_sync(engines[i]) == GASP_OK ? hipSuccess : hipErrorUnknown);
+        // the device header must hold the host's attempt counter before capture
+        GTRY(hipMemcpy(engines[i]->d_rng, &engines[i]->rng, sizeof(RngHeader), hipMemcpyHostToDevice));
+        g->eng.push_back(engines[i]);
+        g->per_replay.push_back((uint64_t)nsweeps[i] * engines[i]->N);
+    }
+    hipEvent_t fork;
+    GTRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    g->ev.push_back(fork);
+    GTRY(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipEventRecord(fork, g->st);
+    int rc = GASP_OK;
+    for (int i = 0; i < n && ce == hipSuccess && !rc; ++i) {
+        gasp_engine *e = engines[i];
+        ce = hipStreamWaitEvent(e->stream, fork, 0);
+        if (ce != hipSuccess) break;
+        const int k = launches ? launches[i] : 1;
+        const uint64_t per = (uint64_t)(nsweeps[i] / k) * e->N;
+        for (int j = 0; j < k && !rc; ++j) {
+            const int chk = (j == k - 1) ? checks[i] : CHECK_NONE;
+            rc = launch_sweep(e, per, nullptr, chk, dn[i], up[i], true);
+        }
+        hipEvent_t join;
+        if (ce == hipSuccess) ce = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+        if (ce == hipSuccess) {
+            g->ev.push_back(join);
+            ce = hipEventRecord(join, e->stream);
+        }
+        if (ce == hipSuccess) ce = hipStreamWaitEvent(g->st, join, 0);
+    }
+    hipError_t ee = hipStreamEndCapture(g->st, &g->graph);
+    if (rc) {
+        gasp_graph_destroy(g);
+        return rc;
+    }
+    GTRY(ce);
+    GTRY(ee);
+    GTRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+#undef GTRY
+    *out = g;
+    return GASP_OK;
+}
+
+extern "C" int gasp_graph_launch(gasp_graph *g, int replays) {
+    clear_error();
+    if (!g || replays < 0) return set_error(GASP_ERR_ARG, "gasp_graph_launch: bad argument");
+    HIP_TRY(hipEventRecord(g->t0, g->st));
+    for (int r = 0; r < replays; ++r) HIP_TRY(hipGraphLaunch(g->exec, g->st));
+    HIP_TRY(hipEventRecord(g->t1, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g->t0, g->t1));
+    g->ms += ms;
+    g->replays += replays;
+    for (size_t i = 0; i < g->eng.size(); ++i) g->eng[i]->rng.attempts += (uint64_t)replays * g->per_replay[i];
+    return GASP_OK;
+}
+
+extern "C" double gasp_graph_ms(gasp_graph *g) { return g ? g->ms : 0.0; }
+
+// ============================================================ autotuner
+// Times each candidate launch shape on a throw-away engine (all spins -1,
+// Philox, probe_sweeps sweeps after one warm-up sweep) and keeps the fastest
+// for (method, L, ngrids).  Candidates: slots per replica S (lanes that
+// speculate on one replica) and, for methods 0/1, waves per workgroup W.
+extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
+                             double *best_rate) {
+    clear_error();
+    if (method < 0 || method > 2 || L < 2 || ngrids < 1 || probe_sweeps < 1)
+        return set_error(GASP_ERR_ARG, "gasp_autotune: bad argument");
+    const int save_s = g_tune_slots, save_w = g_tune_waves;
+    std::vector<std::pair<int, int>> cand;
+    const int smax = method == 2 ? 16 : 64;
+    for (int S = 1; S <= smax; S <<= 1) {
+        if (method == 2) cand.push_back({S, 1});
+        else
+            for (int W = 1; W <= 4; W <<= 1) cand.push_back({S, W});
+    }
+    gasp_rng_state *rng = nullptr;
+    if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();
+    std::vector<int> grids((size_t)ngrids * L * L, -1);
+    float pf[20];
+    uint64_t thr[10];
+    table_f32(0.54, 0.07, pf);  // the reference's metastable regime
+    thr_from_table_f(pf, thr);
+    double best = -1.0;
+    int bs = 1, bw = 1;
+    DeviceInfo di = device_info();
+    for (auto [S, W] : cand) {
+        if (di.ok && lds_bytes_for(method, L * L, S, W) > di.lds_block) continue;
+        g_tune_slots = S;
+        g_tune_waves = W;
+        gasp_engine *e = nullptr;
+        if (engine_new(L, ngrids, method, rng, thr, 0, 0.54, 0.07, &e)) {
+            clear_error();
+            continue;
+        }
+        if (e->S != S || (method != 2 && e->W != W)) {  // shape adjusted by the LDS limit
+            engine_free(e);
+            continue;
+        }
+        double rate = -1.0;
+        if (!engine_set_grids_impl(e, grids.data(), 0, ngrids) && !engine_advance(e, (uint64_t)e->N, 0, 0, 0) &&
+            !engine_sync(e)) {
+            const double ms0 = e->kernel_ms;
+            if (!engine_advance(e, (uint64_t)probe_sweeps * e->N, 0, 0, 0) && !engine_sync(e)) {
+                const double ms = e->kernel_ms - ms0;
+                if (ms > 0) rate = (double)probe_sweeps * e->N * ngrids / (ms * 1e-3);
+            }
+        }
+        engine_free(e);
+        if (rate > best) {
+            best = rate;
+            bs = S;
+            bw = W;
+        }
+    }
+    g_tune_slots = save_s;
+    g_tune_waves = save_w;
+    (void)hipFree(rng);
+    if (best < 0) return set_error(GASP_ERR_ARG, "gasp_autotune: no candidate ran");
+    g_tuned[std::make_tuple(method, L, ngrids)] = {bs, bw};
+    if (best_slots) *best_slots = bs;
+    if (best_waves) *best_waves = bw;
+    if (best_rate) *best_rate = best;
+    return GASP_OK;
+}
 
 // ============================================================ device set-up
 extern "C" int gpuInitDevice(int deviceIndex, int *gpu_nsms) {

@@ -50,6 +50,7 @@ struct SweepArgs {
     int rpw;                 // replicas per workgroup (layouts 0/1)
     int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: words per group
     uint64_t a_begin;        // per-replica attempt index of the first attempt
+    const uint64_t *a_begin_dev;  // if set, a_begin is read from device memory (graph replays)
     uint64_t n_attempts;     // attempts per replica in this launch (< 2^31)
     uint64_t seed;           // Philox key
     const uint32_t *draws;   // RNG_DRAWS: [R][3*n_attempts], draw stream from a_begin
@@ -288,8 +289,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 
     // Launch-local attempt coordinates shifted by b = a_begin & 1 so that
     // attempt pairs (2q, 2q+1) coincide with global Philox pairs P0 + q.
-    const uint32_t b = (uint32_t)(a.a_begin & 1ull);
-    const uint64_t P0 = a.a_begin >> 1;
+    const uint64_t a_begin = a.a_begin_dev ? *a.a_begin_dev : a.a_begin;
+    const uint32_t b = (uint32_t)(a_begin & 1ull);
+    const uint64_t P0 = a_begin >> 1;
     const uint32_t e = (uint32_t)a.n_attempts + b;
     uint32_t l0 = active ? b : e;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
@@ -602,6 +604,16 @@ __global__ void __launch_bounds__(64) mt_draws_kernel(uint32_t *st, int R, uint6
         __syncthreads();
     }
     for (int k = t; k < 624; k += 64) st[(size_t)k * R + r] = mt[k];
+}
+
+// ------------------------------------------------------------ small kernels
+// (kernels rather than memset / host updates, so that they can be captured in
+// a graph and replayed)
+__global__ void zero_ints(int *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+__global__ void advance_counter(uint64_t *c, uint64_t n) {
+    if (threadIdx.x == 0) *c += n;
 }
 
 // ------------------------------------------------------------ checks

@@ -227,6 +227,23 @@ double gasp_engine_kernel_ms(gasp_engine *e);
 /* The engine's HIP stream (hipStream_t). */
 void *gasp_engine_stream(gasp_engine *e);
 
+/* ------------------------------------------------ batched graph, autotuner */
+/* One captured hipGraph advancing n engines concurrently (uneven L, methods,
+ * temperatures and tasks); each replay runs nsweeps[i] sweeps of engine i in
+ * launches[i] kernels (NULL: 1) with threshold check checks[i] at the end.
+ * Counter-based (Philox) engines only.  Engines must outlive the graph; their
+ * getters are valid after gasp_graph_launch returns. */
+typedef struct gasp_graph gasp_graph;
+int gasp_graph_create(gasp_engine **engines, int n, const int *nsweeps, const int *launches,
+                      const int *checks, const double *dn, const double *up, gasp_graph **out);
+int gasp_graph_launch(gasp_graph *g, int replays); /* runs and waits */
+double gasp_graph_ms(gasp_graph *g);               /* GPU time of all replays */
+int gasp_graph_destroy(gasp_graph *g);
+/* Times the candidate launch shapes (slots per replica, waves per workgroup)
+ * for (method, L, ngrids) and keeps the fastest for later engines. */
+int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
+                  double *best_rate);
+
 /* ------------------------------------------------ committor surrogate (CNN) */
 /* No counterpart in the reference: the learned addition.  A 4-layer circular
  * 3x3 CNN (32 channels) + global average pool + MLP (64) + sigmoid mapping an
