@@ -41,21 +41,10 @@ METRIC = "spin-flip attempts/sec (1 GPU, L=64) + committor pB estimates/sec at M
 
 def committor_inputs(n: int, L: int = 64, seed: int = 12345) -> np.ndarray:
     """Committor shooting inputs: a -1 sea with a +1 disc of radius U[4, 14]
-    (sites within periodic distance R of a uniformly random integer centre),
-    then 5% of the sites (uniform, without replacement) flipped."""
-    rs = np.random.default_rng(seed)
-    r = np.arange(L)
-    out = np.empty((n, L, L), dtype=np.int32)
-    for k in range(n):
-        rad = rs.uniform(4.0, 14.0)
-        cy, cx = rs.integers(0, L, 2)
-        dy = np.minimum(np.abs(r - cy), L - np.abs(r - cy))[:, None]
-        dx = np.minimum(np.abs(r - cx), L - np.abs(r - cx))[None, :]
-        g = np.where(dy * dy + dx * dx <= rad * rad, 1, -1).astype(np.int32)
-        flip = rs.choice(L * L, size=int(round(0.05 * L * L)), replace=False)
-        g.reshape(-1)[flip] *= -1
-        out[k] = g
-    return out
+    at a uniformly random integer centre, then 5% of the sites flipped."""
+    from gpu_2dising_amd.surrogate import disc_grids
+
+    return disc_grids(n, L, seed)
 
 
 def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 20000,
@@ -108,6 +97,38 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
         "mean_bootstrap_error": float(np.mean(errs)),
         "workload": "512 grids: -1 sea, +1 disc of radius U[4,14] at a random centre, 5% of spins "
                     "flipped; 1024 replicas each; thresholds -0.90/+0.90; <= 20000 sweeps",
+        "_pB": pB,
+    }
+
+
+SURROGATE_PARAMS = os.path.join(ROOT, "gpu_2dising_amd", "surrogate_L64.npz")
+
+
+def surrogate_leg(G, shooting: dict, n_inputs: int = 512, fallback_reps: int = 128) -> dict:
+    """Committor estimates for the same 512 inputs as the shooting leg from the
+    trained network, with shooting (fallback_reps replicas) for predictions in
+    [0.2, 0.8].  MAE is measured against the shooting leg's 1024-replica pB."""
+    from gpu_2dising_amd import surrogate as SG
+
+    blob = np.load(SURROGATE_PARAMS)
+    params = blob["params"]
+    inputs = committor_inputs(n_inputs, L).astype(np.int8)
+    ref = shooting["_pB"]
+    t0 = time.perf_counter()
+    est, unsure, tm = SG.estimate(inputs, params, reps=fallback_reps, seed=4040)
+    secs = time.perf_counter() - t0
+    ok = ~np.isnan(ref)
+    err = np.abs(est - ref)[ok]
+    return {
+        "metric": "committor pB estimates/sec (surrogate, shooting fallback for pB in [0.2, 0.8])",
+        "value": n_inputs / secs, "unit": "pB estimates/s",
+        "mae_vs_shooting": float(err.mean()),
+        "mae_surrogate_only": float(np.abs(est - ref)[ok & ~unsure].mean()) if (ok & ~unsure).any() else None,
+        "fallback_fraction": float(unsure.mean()), "fallback_replicas": fallback_reps,
+        "seconds": secs, "predict_seconds": tm["predict_seconds"], "fallback_seconds": tm["fallback_seconds"],
+        "speedup_vs_shooting": shooting["gpu_seconds"] / secs,
+        "reference": "pB of the shooting leg (1024 replicas per input)",
+        "training": {k: float(blob[k]) for k in ("val_mae", "n_train") if k in blob.files},
     }
 
 
@@ -224,10 +245,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--method", type=int, default=2)
     ap.add_argument("--slots", type=int, default=0, help="slots per replica (0 = tuner)")
+    ap.add_argument("--word-split", type=int, default=0, choices=(0, 1, 2),
+                    help="replicas per workgroup: 1 = 64 (whole words), 2 = 32 (split words), 0 = tuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the grid dumps")
     ap.add_argument("--no-committor", action="store_true", help="skip the committor shooting leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate committor leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas are independent "
@@ -237,12 +261,12 @@ def main() -> int:
     import gpu_2dising_amd as G
 
     G.init_device(-1)
-    G.set_tuning(args.slots, 0)
+    G.set_tuning(args.slots, args.word_split)
     rng = G.RngState(R, 20251202)
     eng = G.Engine(L, R, args.method, rng, BETA, H)
     g0 = -np.ones((R, L, L), dtype=np.int32)
     host = np.empty((R, L * L), dtype=np.int32)
-    S, _ = G.get_tuning(args.method, L, R)
+    S, split = G.get_tuning(args.method, L, R)
 
     # warmup (untimed), then restart from all -1
     eng.set_grids(g0)
@@ -295,6 +319,7 @@ def main() -> int:
             "L": L, "replicas": R, "sweeps_per_step": SWEEPS_PER_STEP,
             "total_sweeps": args.steps * SWEEPS_PER_STEP, "gpu_method": args.method,
             "rng": "philox4x32-10 per replica", "slots_per_replica": S,
+            "replicas_per_workgroup": (32 if split == 2 else 64) if args.method == 2 else None,
             "grid_dumps": not args.no_dump,
         },
         "roofline": {
@@ -322,6 +347,9 @@ def main() -> int:
     G.set_tuning(0, 0)
     if not args.no_committor:
         out["committor"] = committor_leg(G)
+        if os.path.exists(SURROGATE_PARAMS) and not args.no_surrogate:
+            out["surrogate"] = surrogate_leg(G, out["committor"])
+        out["committor"].pop("_pB")
     if not args.no_batched:
         out["batched"] = batched_leg(G)
     if not args.no_cpu:

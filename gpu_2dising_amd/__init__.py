@@ -29,6 +29,7 @@ EXPORTS = (
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
+    "gasp_engine_gather_grids",
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
@@ -135,6 +136,8 @@ def lib() -> C.CDLL:
     L.gasp_engine_destroy.argtypes = [vp]
     L.gasp_engine_set_grids.argtypes = [vp, vp, C.c_int]
     L.gasp_engine_get_grids.argtypes = [vp, vp, C.c_int]
+    L.gasp_engine_gather_grids.argtypes = [vp, vp, C.c_int, vp]
+    L.gasp_engine_gather_grids.restype = C.c_int
     L.gasp_engine_set_inputs.argtypes = [vp, vp, C.c_int]
     L.gasp_engine_set_freeze.argtypes = [vp, C.c_int]
     L.gasp_engine_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
@@ -240,6 +243,13 @@ class Engine:
     def get_grids(self) -> np.ndarray:
         out = np.empty((self.R, self.L, self.L), dtype=np.int32)
         check(lib().gasp_engine_get_grids(self.ptr, out.ctypes.data, 0))
+        return out
+
+    def gather(self, replicas) -> np.ndarray:
+        """int8 grids [n, L, L] of the listed replicas."""
+        idx = np.ascontiguousarray(replicas, dtype=np.int32)
+        out = np.empty((idx.size, self.L, self.L), dtype=np.int8)
+        check(lib().gasp_engine_gather_grids(self.ptr, idx.ctypes.data, idx.size, out.ctypes.data))
         return out
 
     def sweep(self, nsweeps: int, check_mode: int = CHECK_NONE, dn: float = -0.9,

@@ -35,6 +35,9 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-para
              "-Wno-missing-field-initializers"]
 
 LIB = os.path.join(PKG, "libgasp_amd.so")
+HDF5_LIB = os.path.join(PKG, "libgasp_hdf5.so")
+# HDF5 C library (1.10) used for the Python module's grid files
+HDF5_ROOT = os.environ.get("GASP_HDF5_ROOT", "/opt/conda")
 CLI = os.path.join(BIN, "gasp")
 EXT_SUFFIX = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
 PYEXT = os.path.join(PKG, "gasp" + EXT_SUFFIX)
@@ -59,6 +62,17 @@ def build_lib() -> str:
     return LIB
 
 
+def build_hdf5() -> str:
+    """libgasp_hdf5.so: create/write_ising_grids_hdf5 (include/gasp_hdf5.h)."""
+    tmp = HDF5_LIB + ".tmp"
+    _run([CXX, *CXX_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "gasp_hdf5.cpp"),
+          # after the system directories, so that only hdf5*.h come from there
+          "-idirafter", os.path.join(HDF5_ROOT, "include"),
+          f"-L{os.path.join(HDF5_ROOT, 'lib')}", "-lhdf5", f"-Wl,-rpath,{os.path.join(HDF5_ROOT, 'lib')}"])
+    os.replace(tmp, HDF5_LIB)
+    return HDF5_LIB
+
+
 def build_cli() -> str:
     os.makedirs(BIN, exist_ok=True)
     _run([CXX, *CXX_FLAGS, "-o", CLI, os.path.join(CSRC, "gasp_cli.cpp"),
@@ -72,7 +86,7 @@ def build_pyext() -> str:
     py_inc = sysconfig.get_paths()["include"]
     tmp = PYEXT + ".tmp"
     _run([CXX, *CXX_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "gasp_pymod.cpp"),
-          f"-I{py_inc}", f"-I{np.get_include()}", f"-L{PKG}", "-lgasp_amd",
+          f"-I{py_inc}", f"-I{np.get_include()}", f"-L{PKG}", "-lgasp_amd", "-lgasp_hdf5",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, PYEXT)
     return PYEXT
@@ -80,6 +94,7 @@ def build_pyext() -> str:
 
 def build_all() -> None:
     build_lib()
+    build_hdf5()
     if os.path.exists(os.path.join(CSRC, "gasp_cli.cpp")):
         build_cli()
     if os.path.exists(os.path.join(CSRC, "gasp_pymod.cpp")):
@@ -89,4 +104,4 @@ def build_all() -> None:
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "cli": build_cli, "pyext": build_pyext}[w]()
+        {"all": build_all, "lib": build_lib, "hdf5": build_hdf5, "cli": build_cli, "pyext": build_pyext}[w]()

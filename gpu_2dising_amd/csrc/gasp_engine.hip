@@ -232,9 +232,10 @@ static int layout_stride(int method, int N) {
     return ((N + 15) / 16) * 16;                  // bytes per replica
 }
 
+// Method 2: `waves` is the word split (1: 64 replicas per workgroup, 2: 32).
 static size_t lds_bytes_for(int method, int N, int S, int waves) {
     const int stride = layout_stride(method, N);
-    if (method == 2) return (size_t)kLdsLattice + 8 * (size_t)stride;
+    if (method == 2) return (size_t)kLdsLattice + (waves == 2 ? 4 : 8) * (size_t)stride;
     if (method == 1) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
@@ -262,8 +263,10 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     const int ncu = d.ok ? d.ncu : 256;
     const size_t lds_block = d.ok ? d.lds_block : 65536;
     const int N = L * L;
-    int W = (method == 2) ? 1 : (g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1);
-    int Smax = (method == 2) ? 16 : 64;
+    int W;
+    if (method == 2) W = (g_tune_waves == 2 || g_tune_slots > 16) ? 2 : 1;
+    else W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
+    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : 64;
     int S;
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), Smax);
@@ -312,6 +315,8 @@ static SweepFn pick_sweep(int S) {
     }
     if constexpr (LAT != LAT_MULTISPIN) {
         if (S == 32) return sweep_kernel<LAT, RNG, 32, P2>;
+    }
+    if constexpr (LAT != LAT_MULTISPIN && LAT != LAT_MULTIHALF) {
         if (S == 64) return sweep_kernel<LAT, RNG, 64, P2>;
     }
     return nullptr;
@@ -322,7 +327,9 @@ static SweepFn pick_sweep2(int S, bool pow2) {
     return pow2 ? pick_sweep<LAT, RNG, true>(S) : pick_sweep<LAT, RNG, false>(S);
 }
 
-static SweepFn sweep_fn(int method, int rng, int S, bool pow2) {
+static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2) {
+    if (method == 2 && W == 2)
+        return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2);
     if (method == 2)
         return rng == 0 ? pick_sweep2<LAT_MULTISPIN, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTISPIN, RNG_DRAWS>(S, pow2);
     if (method == 1)
@@ -429,10 +436,11 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     DeviceInfo di = device_info();
     e->lds = lds_bytes_for(method, e->N, e->S, e->W);
     if (method == 2) {
-        e->block = 64 * e->S;
-        e->nblocks = (R + 63) / 64;
-        e->lat_bytes = (size_t)e->nblocks * e->stride * 8;
-        e->rpw = 64;
+        const int groups = (R + 63) / 64;
+        e->rpw = e->W == 2 ? 32 : 64;
+        e->block = e->rpw * e->S;
+        e->nblocks = groups * (64 / e->rpw);
+        e->lat_bytes = (size_t)groups * e->stride * 8;
     } else {
         e->block = 64 * e->W;
         e->rpw = std::max(1, e->W * 64 / e->S);
@@ -447,7 +455,7 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
                          "limit %zu); use select_gpu_method",
                          method, need, L, (size_t)di.lds_block);
     }
-    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, is_pow2(L));
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L));
     if (!fn) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
@@ -532,7 +540,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.freeze = e->freeze;
     a.dn_thr = dn;
     a.up_thr = up;
-    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, is_pow2(e->L));
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L));
     if (check != CHECK_NONE) {
         hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
         HIP_TRY(hipGetLastError());
@@ -802,6 +810,36 @@ extern "C" int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device) 
     return rc ? rc : engine_sync(e);
 }
 
+extern "C" int gasp_engine_gather_grids(gasp_engine *e, const int *replicas, int n, int8_t *host_out) {
+    clear_error();
+    if (!e || (n > 0 && (!replicas || !host_out)) || n < 0)
+        return set_error(GASP_ERR_ARG, "gasp_engine_gather_grids: bad argument");
+    if (n == 0) return GASP_OK;
+    for (int k = 0; k < n; ++k)
+        if (replicas[k] < 0 || replicas[k] >= e->R)
+            return set_error(GASP_ERR_ARG, "gasp_engine_gather_grids: replica %d out of range [0, %d)", replicas[k],
+                             e->R);
+    int *d_idx = nullptr;
+    int8_t *d_out = nullptr;
+    const size_t bytes = (size_t)n * e->N;
+    hipError_t he = hipMalloc(&d_idx, (size_t)n * sizeof(int));
+    if (he == hipSuccess) he = hipMalloc(&d_out, bytes);
+    if (he == hipSuccess) he = hipMemcpyAsync(d_idx, replicas, (size_t)n * sizeof(int), hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(gather_spins8, dim3(1024), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+                           e->stride, d_idx, n, e->N, d_out);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(host_out, d_out, bytes, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    dfree(d_idx);
+    dfree(d_out);
+    if (he != hipSuccess)
+        return set_error(he == hipErrorOutOfMemory ? GASP_ERR_NOMEM : GASP_ERR_HIP, "GPUassert: %s %s %d",
+                         hipGetErrorString(he), __FILE__, __LINE__);
+    return engine_collect_events(e);
+}
+
 extern "C" int gasp_engine_sweep(gasp_engine *e, int nsweeps, int check, double dn_thr, double up_thr) {
     clear_error();
     if (!e) return set_error(GASP_ERR_ARG, "gasp_engine_sweep: NULL engine");
@@ -988,11 +1026,13 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
         return set_error(GASP_ERR_ARG, "gasp_autotune: bad argument");
     const int save_s = g_tune_slots, save_w = g_tune_waves;
     std::vector<std::pair<int, int>> cand;
-    const int smax = method == 2 ? 16 : 64;
-    for (int S = 1; S <= smax; S <<= 1) {
-        if (method == 2) cand.push_back({S, 1});
-        else
+    for (int S = 1; S <= 64; S <<= 1) {
+        if (method == 2) {
+            if (S <= 16) cand.push_back({S, 1});
+            if (S >= 2 && S <= 32) cand.push_back({S, 2});
+        } else {
             for (int W = 1; W <= 4; W <<= 1) cand.push_back({S, W});
+        }
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();
@@ -1013,7 +1053,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
             clear_error();
             continue;
         }
-        if (e->S != S || (method != 2 && e->W != W)) {  // shape adjusted by the LDS limit
+        if (e->S != S || e->W != W) {  // shape adjusted by the LDS limit
             engine_free(e);
             continue;
         }

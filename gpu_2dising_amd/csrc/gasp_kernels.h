@@ -33,15 +33,17 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 constexpr uint32_t kPhiloxDomain = 0x47415350u;  // "GASP"
 
-enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2 };
+enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3 };
 enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
 enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
-// LDS header shared by all layouts: 10 thresholds (80 B, padded to 128) and
-// 64 per-replica counters.
+// LDS header shared by all layouts: 10 thresholds (80 B, padded to 128),
+// 64 per-replica counters and the 64 row/column neighbourhood masks of the
+// conflict filter.
 constexpr int kLdsThr = 0;
 constexpr int kLdsCnt = 128;
-constexpr int kLdsLattice = 384;
+constexpr int kLdsNbr = 384;
+constexpr int kLdsLattice = 896;
 
 struct SweepArgs {
     void *lattice;           // native layout (see engine)
@@ -88,6 +90,7 @@ __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32
 //  which for L = 2^lg is u >> (32 - lg).
 struct Stencil {
     int c, up, dn, lt, rt;
+    int row, col;
 };
 __device__ __forceinline__ bool in_stencil(int x, const Stencil &s) {
     return (x == s.c) | (x == s.up) | (x == s.dn) | (x == s.lt) | (x == s.rt);
@@ -99,6 +102,8 @@ struct Geometry {
 
     __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {
         Stencil s;
+        s.row = row;
+        s.col = col;
         s.c = row * L + col;
         s.up = (row == 0) ? s.c + (N - L) : s.c - L;
         s.dn = (row == L - 1) ? s.c - (N - L) : s.c + L;
@@ -109,6 +114,8 @@ struct Geometry {
     __device__ __forceinline__ Stencil from_site_pow2(int c) const {
         const int m = N - 1, cm = L - 1;
         Stencil s;
+        s.row = c >> lg;
+        s.col = c & cm;
         s.c = c;
         s.up = (c - L) & m;
         s.dn = (c + L) & m;
@@ -141,6 +148,9 @@ struct Geometry {
 // Layout 2 (multispin): uint64 word per site, bit b = replica (64*group + b).
 //   Lane reads the byte holding its bit; flips with an LDS atomic XOR on the
 //   dword holding it (other waves flip other bits of the same word).
+// Layout 3 (multispin, split word): the same HBM words, but a workgroup holds
+//   one 32-bit half of each (32 replicas): a dword per site in LDS, half the
+//   LDS of layout 2, so twice the workgroups for the same replicas.
 // Layout 1 (bit lattice): replica-private region of `stride` dwords in LDS,
 //   bit (site & 31) of dword (site >> 5).
 // Layout 0 (HBM): one byte (0/1) per site in global memory, replica stride
@@ -155,6 +165,8 @@ struct Lattice {
     __device__ __forceinline__ int spin(int site) const {
         if constexpr (LAT == LAT_MULTISPIN) {
             return (base[site * 8] >> (rl & 7)) & 1;
+        } else if constexpr (LAT == LAT_MULTIHALF) {
+            return (base[site * 4] >> (rl & 7)) & 1;
         } else if constexpr (LAT == LAT_BITS) {
             return (base[site >> 3] >> (site & 7)) & 1;
         } else {
@@ -165,6 +177,8 @@ struct Lattice {
         if constexpr (LAT == LAT_MULTISPIN) {
             // word `site` is dwords 2*site (bits 0..31) and 2*site+1 (bits 32..63)
             atomicXor(dw + 2 * site + (rl >> 5), 1u << (rl & 31));
+        } else if constexpr (LAT == LAT_MULTIHALF) {
+            atomicXor(dw + site, 1u << (rl & 31));
         } else if constexpr (LAT == LAT_BITS) {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
         } else {
@@ -190,6 +204,60 @@ __device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stenc
     const uint64_t T = thr_lds[5 * t.s01 + nup];
     t.acc = en && ((uint64_t)ua < T);
     return t;
+}
+
+// ------------------------------------------------------------ conflict filter
+// Bits of the rows (or columns) within periodic distance 1 of t, folded mod
+// 64.  Exact for L <= 64; for power-of-two L > 64, entry (r & 63) holds the
+// fold of the rows around r (64 divides L, so (r +- 1 mod L) & 63 =
+// ((r & 63) +- 1) & 63).  An attempt at (row, col) can only conflict with an
+// accepted attempt whose row mask holds `row` and whose column mask holds
+// `col`.
+__device__ __forceinline__ uint64_t nbr_mask(int t, int L) {
+    int rm1, rp1;
+    if (L <= 64) {
+        if (t >= L) return 0ull;
+        rm1 = (t == 0) ? L - 1 : t - 1;
+        rp1 = (t == L - 1) ? 0 : t + 1;
+    } else {
+        rm1 = (t - 1) & 63;
+        rp1 = (t + 1) & 63;
+    }
+    return (1ull << rm1) | (1ull << t) | (1ull << rp1);
+}
+
+// DPP row_shr:K (lane l reads lane l - K of its 16-lane row; 0 outside it).
+template <int K>
+__device__ __forceinline__ uint32_t row_shr(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + K, 0xf, 0xf, true);
+}
+
+template <int S, int K>
+__device__ __forceinline__ void scan_step(uint32_t &lo, uint32_t &hi, int slot) {
+    uint32_t tl = row_shr<K>(lo), th = row_shr<K>(hi);
+    if constexpr (S < 16) {
+        // the source lane belongs to the previous group
+        if (slot < K) tl = th = 0u;
+    }
+    lo |= tl;
+    hi |= th;
+}
+
+// OR of x over the lanes of the group (S <= 16 consecutive lanes, aligned)
+// that precede this lane.
+template <int S>
+__device__ __forceinline__ uint64_t group_excl_or(uint64_t x, int slot) {
+    static_assert(S >= 2 && S <= 16, "DPP scan within a 16-lane row");
+    uint32_t lo = row_shr<1>((uint32_t)x), hi = row_shr<1>((uint32_t)(x >> 32));
+    if constexpr (S < 16) {
+        if (slot == 0) lo = hi = 0u;
+    }
+    // inclusive OR-scan of the shifted values (Hillis-Steele: steps k < S)
+    scan_step<S, 1>(lo, hi, slot);
+    if constexpr (S > 2) scan_step<S, 2>(lo, hi, slot);
+    if constexpr (S > 4) scan_step<S, 4>(lo, hi, slot);
+    if constexpr (S > 8) scan_step<S, 8>(lo, hi, slot);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Per-replica epilogue: magnetisation and the threshold check.  Must be
@@ -230,9 +298,14 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
 template <int LAT, int RNG, int S, bool POW2>
 __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
+    constexpr bool MULTI = (LAT == LAT_MULTISPIN) || (LAT == LAT_MULTIHALF);
+    // replicas per workgroup in the multispin layouts; their lattice word half
+    constexpr int RPG = (LAT == LAT_MULTIHALF) ? 32 : 64;
+    static_assert(!MULTI || RPG * S <= 1024, "workgroup too large");
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
     unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);
+    uint64_t *nbr_lds = (uint64_t *)(smem + kLdsNbr);
     unsigned char *lat_lds = smem + kLdsLattice;
 
     const int tid = threadIdx.x;
@@ -243,9 +316,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     const int gbase = lane & ~(S - 1);
 
     int rl, r0;
-    if constexpr (LAT == LAT_MULTISPIN) {
+    if constexpr (MULTI) {
         rl = wave * (64 / S) + (lane / S);
-        r0 = blockIdx.x * 64;
+        r0 = (LAT == LAT_MULTIHALF) ? (int)(blockIdx.x >> 1) * 64 + 32 * (int)(blockIdx.x & 1) : blockIdx.x * 64;
     } else {
         rl = tid / S;
         r0 = blockIdx.x * a.rpw;
@@ -261,12 +334,21 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     for (int i = 0; i < 10; ++i)
         if (tid == i) thr_lds[i] = a.thr[i];
     if (tid < 64) cnt_lds[tid] = 0u;
+    for (int t = tid; t < 64; t += blockDim.x) nbr_lds[t] = nbr_mask(t, a.L);
+    // the row / column filter needs exact (L <= 64) or aligned (64 | L) folding
+    const bool use_filter = POW2 || a.L <= 64;
     if constexpr (LAT == LAT_MULTISPIN) {
         // a.stride words per group (N rounded up to even): 16-byte aligned
         const uint4 *src = (const uint4 *)((const uint64_t *)a.lattice + (size_t)blockIdx.x * a.stride);
         uint4 *dst = (uint4 *)lat_lds;
         const int n16 = a.stride / 2;
         for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+    } else if constexpr (LAT == LAT_MULTIHALF) {
+        // dword (2 site + half) of the group's words -> LDS dword `site`
+        const unsigned *src = (const unsigned *)((const uint64_t *)a.lattice + (size_t)(blockIdx.x >> 1) * a.stride) +
+                              (blockIdx.x & 1);
+        unsigned *dst = (unsigned *)lat_lds;
+        for (int i = tid; i < a.stride; i += blockDim.x) dst[i] = src[2 * i];
     } else if constexpr (LAT == LAT_BITS) {
         const int nrep = min(a.rpw, a.R - r0);
         const int ndw = nrep * a.stride;
@@ -279,7 +361,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     Lattice<LAT> lat;
     lat.dw = (unsigned *)lat_lds;
     lat.rl = rl;
-    if constexpr (LAT == LAT_MULTISPIN) {
+    if constexpr (MULTI) {
         lat.base = lat_lds + (rl >> 3);
     } else if constexpr (LAT == LAT_BITS) {
         lat.base = lat_lds + (size_t)rl * a.stride * 4;
@@ -336,34 +418,56 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         const Attempt A = evaluate<LAT>(lat, sA, uaA, enA, thr_lds);
         const Attempt B = evaluate<LAT>(lat, sB, uaB, enB, thr_lds);
 
-        bool conflA = false;
-        bool conflB = A.acc && in_stencil(A.st.c, B.st);
+        // B conflicts with the same lane's accepted A
+        const bool ownB = A.acc && in_stencil(A.st.c, B.st);
         bool validA, validB;
         if constexpr (S > 1) {
+            // Wave masks: cA / cB = lanes whose attempt A / B has an earlier
+            // accepted attempt of another lane of its group in its stencil.
+            // The loop runs over the accepted attempts only (wave-uniform);
+            // `later` (the lanes after i in i's group) is a scalar mask.
             const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
+            uint64_t cA = 0ull, cB = __ballot(ownB);
             uint64_t m = mA | mB;
+            if constexpr (S <= 16) {
+                // Filter: OR the row / column neighbourhood masks of the
+                // accepted attempts of the lanes before this one in the
+                // group; if no attempt of the wave has both its row and its
+                // column covered, no attempt conflicts and the exact loop is
+                // skipped (it runs unchanged otherwise).
+                if (use_filter && m != 0ull) {
+                    const uint64_t rA = nbr_lds[A.st.row & 63], kA = nbr_lds[A.st.col & 63];
+                    const uint64_t rB = nbr_lds[B.st.row & 63], kB = nbr_lds[B.st.col & 63];
+                    const uint64_t rm = (A.acc ? rA : 0ull) | (B.acc ? rB : 0ull);
+                    const uint64_t km = (A.acc ? kA : 0ull) | (B.acc ? kB : 0ull);
+                    const uint64_t er = group_excl_or<S>(rm, slot), ek = group_excl_or<S>(km, slot);
+                    const bool sA = ((er >> (A.st.row & 63)) & (ek >> (A.st.col & 63)) & 1ull) != 0ull;
+                    const bool sB = ((er >> (B.st.row & 63)) & (ek >> (B.st.col & 63)) & 1ull) != 0ull;
+                    if (__ballot(sA || sB) == 0ull) m = 0ull;
+                }
+            }
             while (m) {
                 const int i = __builtin_ctzll(m);
                 m &= m - 1;
-                const bool later = ((lane ^ i) < S) && (lane > i);
+                const uint64_t grp = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << (i & ~(S - 1)));
+                const uint64_t later = grp & ~((2ull << i) - 1ull);  // i = 63: 2 << 63 = 0, later = 0
                 if ((mA >> i) & 1ull) {
                     const int x = __builtin_amdgcn_readlane(A.st.c, i);
-                    conflA |= later && in_stencil(x, A.st);
-                    conflB |= later && in_stencil(x, B.st);
+                    cA |= __ballot(in_stencil(x, A.st)) & later;
+                    cB |= __ballot(in_stencil(x, B.st)) & later;
                 }
                 if ((mB >> i) & 1ull) {
                     const int x = __builtin_amdgcn_readlane(B.st.c, i);
-                    conflA |= later && in_stencil(x, A.st);
-                    conflB |= later && in_stencil(x, B.st);
+                    cA |= __ballot(in_stencil(x, A.st)) & later;
+                    cB |= __ballot(in_stencil(x, B.st)) & later;
                 }
             }
-            const uint64_t cA = __ballot(conflA), cB = __ballot(conflB);
             const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << gbase) - 1ull);
-            validA = (((cA | cB) & below) == 0ull) && !conflA;
-            validB = validA && !conflB;
+            validA = (((cA | cB) & below) == 0ull) && !((cA >> lane) & 1ull);
+            validB = validA && !((cB >> lane) & 1ull);
         } else {
             validA = true;
-            validB = !conflB;
+            validB = !ownB;
         }
 
         if (A.acc && validA) lat.flip(A.st.c, A.s01);
@@ -416,6 +520,20 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         const uint4 *src = (const uint4 *)lat_lds;
         const int n16 = a.stride / 2;
         for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+    } else if constexpr (LAT == LAT_MULTIHALF) {
+        __syncthreads();
+        // thread t counts bit (t & 31) over sites t>>5, t>>5 + S, ...
+        const int b = tid & 31;
+        const int q = tid >> 5;
+        const unsigned *w = (const unsigned *)lat_lds;
+        unsigned c = 0;
+        for (int site = q; site < N; site += S) c += (w[site] >> b) & 1u;
+        atomicAdd(&cnt_lds[b], c);
+        __syncthreads();
+        if (tid < 32) finish_replica(a, (r0 + tid) < a.R, r0 + tid, cnt_lds[tid]);
+        unsigned *dst = (unsigned *)((uint64_t *)a.lattice + (size_t)(blockIdx.x >> 1) * a.stride) + (blockIdx.x & 1);
+        const unsigned *src = (const unsigned *)lat_lds;
+        for (int i = tid; i < a.stride; i += blockDim.x) dst[2 * i] = src[i];
     } else {
         if constexpr (LAT == LAT_BITS) {
             __syncthreads();
@@ -609,6 +727,25 @@ __global__ void __launch_bounds__(64) mt_draws_kernel(uint32_t *st, int R, uint6
 // ------------------------------------------------------------ small kernels
 // (kernels rather than memset / host updates, so that they can be captured in
 // a graph and replayed)
+// Selected replicas idx[0..n) as int8 +-1 grids [n][N], from any native layout.
+__global__ void gather_spins8(const void *__restrict__ lat, int method, int stride, const int *__restrict__ idx,
+                              int n, int N, int8_t *__restrict__ out) {
+    const size_t total = (size_t)n * N;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / N), site = (int)(i - (size_t)k * N);
+        const int r = idx[k];
+        int up;
+        if (method == 2) {
+            up = (int)((((const uint64_t *)lat)[(size_t)(r >> 6) * stride + site] >> (r & 63)) & 1ull);
+        } else if (method == 1) {
+            up = (int)((((const unsigned *)lat)[(size_t)r * stride + (site >> 5)] >> (site & 31)) & 1u);
+        } else {
+            up = ((const unsigned char *)lat)[(size_t)r * stride + site];
+        }
+        out[i] = up ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
 __global__ void zero_ints(int *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }

@@ -17,8 +17,10 @@
 // gridinput.bin instead of dereferencing a missing list; the GIL is released
 // while the GPU runs and re-taken for output; keyword `seed` (default: the
 // clock, as the reference) and `rng` ('philox' or 'mt19937') are added;
-// without a GPU the run functions raise instead of falling back to a CPU path.
-// HDF5 output (gridstates.hdf5 / pBgrids.hdf5) is not written.
+// without a GPU the run functions raise instead of falling back to a CPU path;
+// keyword `write_hdf5` (default True, as the reference always writes) can skip
+// the HDF5 grid file (gridstates.hdf5 / pBgrids.hdf5, written through
+// libgasp_hdf5 in the reference's layout).
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #include <structmember.h>
@@ -35,6 +37,7 @@
 #include <vector>
 
 #include "../../include/gasp_amd.h"
+#include "../../include/gasp_hdf5.h"
 
 extern "C" void gasp_set_output(void (*cb)(const char *text), int style);
 
@@ -138,6 +141,7 @@ static PyObject *GridSnap_deepcopy(PyObject *self, PyObject *) {
 
 // =================================================================== run state
 struct RunState {
+    bool write_hdf5 = true;
     bool keep_grids = false;
     long long maxhist = 0;
     long long ihist = 0;
@@ -194,9 +198,10 @@ static int append_to_module_list(const char *name, PyObject *item) {
 }
 
 // Output callback (src/ising_ext.cu:280-463): module lists of magnetisation and
-// largest cluster, and GridSnapObjects for replicas with dn_thr < cv < up_thr.
+// largest cluster, every grid to the HDF5 file, and GridSnapObjects for
+// replicas with dn_thr < cv < up_thr.
 static int append_grids_list_body(int L, int ngrids, int *grid_data, int isweep, float *magnetisation,
-                                  float *lclus_size, char *cv, double dn_thr, double up_thr) {
+                                  float *lclus_size, char *cv, double dn_thr, double up_thr, char *filename) {
     const float *cv_val = nullptr;
     if (strcmp(cv, "magnetisation") == 0) cv_val = magnetisation;
     else if (strcmp(cv, "largest_cluster") == 0) cv_val = lclus_size;
@@ -215,6 +220,12 @@ static int append_grids_list_body(int L, int ngrids, int *grid_data, int isweep,
         rc = append_to_module_list("largest_cluster", cl);
         Py_DECREF(cl);
         if (rc < 0) return -1;
+    }
+    if (g_run.write_hdf5 &&
+        write_ising_grids_hdf5(L, ngrids, grid_data, isweep, magnetisation, lclus_size, cv, dn_thr, up_thr,
+                               filename) != 0) {
+        PyErr_SetString(PyExc_RuntimeError, "Error writing HDF5 file");
+        return -1;
     }
     if (!g_run.keep_grids || g_run.ihist >= g_run.maxhist) return 0;
 
@@ -270,11 +281,11 @@ static int append_grids_list_body(int L, int ngrids, int *grid_data, int isweep,
 
 extern "C" int append_grids_list(int L, int ngrids, int *grid_data, int isweep, float *magnetisation,
                                  float *lclus_size, char *cv, double dn_thr, double up_thr, char *filename) {
-    (void)filename;
     PyGILState_STATE st = PyGILState_Ensure();
     int rc = 0;
     if (!g_run.failed) {
-        rc = append_grids_list_body(L, ngrids, grid_data, isweep, magnetisation, lclus_size, cv, dn_thr, up_thr);
+        rc = append_grids_list_body(L, ngrids, grid_data, isweep, magnetisation, lclus_size, cv, dn_thr, up_thr,
+                                    filename);
         if (rc < 0) {
             g_run.failed = true;
             PyErr_Fetch(&g_run.err_type, &g_run.err_value, &g_run.err_tb);
@@ -365,6 +376,9 @@ static bool run_driver(int L, int ngrids, int *ising_grids, int *grid_fate, doub
 
     g_run.failed = false;
     g_run.err_type = g_run.err_value = g_run.err_tb = nullptr;
+    // src/ising_ext.cu:660, 700, 1026, 1062 (return value unchecked there too:
+    // a file that cannot be created fails at the first grid output)
+    if (g_run.write_hdf5) create_ising_grids_hdf5(L, ngrids, tot_nsweeps, h, beta, itask, &fname[0]);
     gasp_set_output(py_print, 1);
     int rc;
     Py_BEGIN_ALLOW_THREADS
@@ -404,7 +418,8 @@ static bool begin_history(int L, int ngrids, int tot_nsweeps, int grid_output_in
 static const char *nuc_doc =
     "run_nucleation_swarm(L, ngrid, tot_nsweeps, beta, h, initial_spin=-1, cv='magnetisation', up_threshold=0.9,\n"
     "                     dn_threshold=-0.95, mag_output_int=100, grid_output_int=1000, keep_grids=True,\n"
-    "                     max_keep_grids=0, threadsPerBlock=32, gpu_method=None, seed=None, rng='philox')\n"
+    "                     max_keep_grids=0, threadsPerBlock=32, gpu_method=None, seed=None, rng='philox',\n"
+    "                     write_hdf5=True)\n"
     "\n"
     "Simulate a swarm of 2D Ising replicas from a uniform initial_spin state on the GPU.  Returns the\n"
     "fraction of replicas whose collective variable exceeds up_threshold at each mag_output_int\n"
@@ -428,12 +443,15 @@ static PyObject *run_nucleation_swarm(PyObject *, PyObject *args, PyObject *kwar
     const char *rng = "philox";
     static const char *kwlist[] = {"L", "ngrid", "tot_nsweeps", "beta", "h", "initial_spin", "cv", "up_threshold",
                                    "dn_threshold", "mag_output_int", "grid_output_int", "keep_grids",
-                                   "max_keep_grids", "threadsPerBlock", "gpu_method", "seed", "rng", nullptr};
-    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "iiidd|isddiipiiOOs", (char **)kwlist, &L, &ngrids,
+                                   "max_keep_grids", "threadsPerBlock", "gpu_method", "seed", "rng",
+                                   "write_hdf5", nullptr};
+    int write_hdf5 = 1;
+    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "iiidd|isddiipiiOOsp", (char **)kwlist, &L, &ngrids,
                                      &tot_nsweeps, &beta, &h, &initial_spin, &cv, &up_threshold, &dn_threshold,
                                      &mag_output_int, &grid_output_int, &keep_grids, &max_keep_grids, &tpb,
-                                     &method_obj, &seed_obj, &rng))
+                                     &method_obj, &seed_obj, &rng, &write_hdf5))
         return nullptr;
+    g_run.write_hdf5 = write_hdf5 != 0;
     if (strcmp(cv, "magnetisation") != 0 && strcmp(cv, "largest_cluster") != 0) {
         PyErr_SetString(PyExc_ValueError, "cv must be either 'magnetisation' or 'largest_cluster'");
         return nullptr;
@@ -479,7 +497,7 @@ static const char *comm_doc =
     "run_committor_calc(L, ngrid, tot_nsweeps, beta, h, initial_spin=-1, cv='magnetisation', up_threshold=0.9,\n"
     "                   dn_threshold=-0.93, mag_output_int=100, grid_output_int=1000, keep_grids=True,\n"
     "                   max_keep_grids=0, threadsPerBlock=32, gpu_method=None, grid_input='gridstates.bin',\n"
-    "                   grid_array=None, nsms=gpu_nsms, seed=None, rng='philox')\n"
+    "                   grid_array=None, nsms=gpu_nsms, seed=None, rng='philox', write_hdf5=True)\n"
     "\n"
     "Committor by shooting: each input grid (grid_input='NumPy' with grid_array a list of LxL int8\n"
     "arrays, or gridinput.bin) is copied into ngrid // len(grid_array) replicas, which run until their\n"
@@ -501,12 +519,15 @@ static PyObject *run_committor_calc(PyObject *, PyObject *args, PyObject *kwargs
     static const char *kwlist[] = {"L", "ngrid", "tot_nsweeps", "beta", "h", "initial_spin", "cv", "up_threshold",
                                    "dn_threshold", "mag_output_int", "grid_output_int", "keep_grids",
                                    "max_keep_grids", "threadsPerBlock", "gpu_method", "grid_input", "grid_array",
-                                   "nsms", "seed", "rng", nullptr};
-    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "iiidd|isddiipiiOsOiOs", (char **)kwlist, &L, &ngrids,
+                                   "nsms", "seed", "rng", "write_hdf5", nullptr};
+    int write_hdf5 = 1;
+    if (!PyArg_ParseTupleAndKeywords(args, kwargs, "iiidd|isddiipiiOsOiOsp", (char **)kwlist, &L, &ngrids,
                                      &tot_nsweeps, &beta, &h, &initial_spin, &cv, &up_threshold, &dn_threshold,
                                      &mag_output_int, &grid_output_int, &keep_grids, &max_keep_grids, &tpb,
-                                     &method_obj, &grid_input, &grid_array_obj, &nsms, &seed_obj, &rng))
+                                     &method_obj, &grid_input, &grid_array_obj, &nsms, &seed_obj, &rng,
+                                     &write_hdf5))
         return nullptr;
+    g_run.write_hdf5 = write_hdf5 != 0;
     // src/ising_ext.cu:852-900
     if (nsms < 0) {
         PyErr_SetString(PyExc_ValueError, "nsms must be a non-negative integer");

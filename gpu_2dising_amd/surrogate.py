@@ -4,8 +4,10 @@ estimation with shooting as the fallback.
 Pipeline (defaults follow the benchmark specification at L = 64):
   1. sample_trajectories: a nucleation swarm (all spins -1, beta = 0.54,
      h = 0.07, Philox streams); grids are sampled every `every` sweeps.
-  2. select: 20000 grids, stratified by magnetisation towards the transition
-     region (grids outside the thresholds have pB 0 or 1 by definition).
+  2. select: 14000 grids, stratified by magnetisation over the transition
+     region (grids outside the thresholds have pB 0 or 1 by definition), plus
+     6000 committor-input-style grids (disc_grids: a +1 disc in a -1 sea with
+     5% of the spins flipped) so that the network also covers shooting inputs.
   3. label_by_shooting: each grid gets `reps` replicas (256) run until every
      fate is decided against dn/up thresholds; the label is nB / (nA + nB).
   4. train: the CNN of gasp_cnn_* (BF16 MFMA kernels, Adam, batch 256) on a
@@ -30,8 +32,28 @@ BETA, H = 0.54, 0.07
 DN, UP = -0.90, 0.90
 
 
+def disc_grids(n: int, L: int = 64, seed: int = 12345, r_lo: float = 4.0, r_hi: float = 14.0,
+               flip: float = 0.05) -> np.ndarray:
+    """Committor shooting inputs: a -1 sea with a +1 disc of radius U[r_lo, r_hi]
+    (sites within periodic distance R of a uniformly random integer centre),
+    then a fraction `flip` of the sites (uniform, without replacement) flipped."""
+    rs = np.random.default_rng(seed)
+    r = np.arange(L)
+    out = np.empty((n, L, L), dtype=np.int32)
+    for k in range(n):
+        rad = rs.uniform(r_lo, r_hi)
+        cy, cx = rs.integers(0, L, 2)
+        dy = np.minimum(np.abs(r - cy), L - np.abs(r - cy))[:, None]
+        dx = np.minimum(np.abs(r - cx), L - np.abs(r - cx))[None, :]
+        g = np.where(dy * dy + dx * dx <= rad * rad, 1, -1).astype(np.int32)
+        fl = rs.choice(L * L, size=int(round(flip * L * L)), replace=False)
+        g.reshape(-1)[fl] *= -1
+        out[k] = g
+    return out
+
+
 def sample_trajectories(L=64, R=16384, sweeps=50000, every=500, seed=101, keep_per_snapshot=1024,
-                        window=(-0.88, 0.88), rng_seed=0):
+                        window=(-0.88, 0.88), rng_seed=0, log=None):
     """Returns int8 grids, their magnetisation and sweep.  At every sample all
     replicas inside `window` are kept, plus `keep_per_snapshot` others drawn
     at random (metastable or nucleated)."""
@@ -47,28 +69,44 @@ def sample_trajectories(L=64, R=16384, sweeps=50000, every=500, seed=101, keep_p
             others = np.setdiff1d(np.arange(R), inside)
             extra = rs.choice(others, size=min(keep_per_snapshot, others.size), replace=False)
             idx = np.concatenate([inside, extra])
-            g = e.get_grids()[idx].astype(np.int8)
+            g = e.gather(idx)
             grids.append(g)
             mags.append(m[idx])
             times.append(np.full(idx.size, (k + 1) * every))
+            if log and (k + 1) % 10 == 0:
+                log(f"sampled to sweep {(k + 1) * every}: {sum(len(x) for x in grids)} grids")
     st.free()
     return np.concatenate(grids), np.concatenate(mags), np.concatenate(times)
 
 
-def select(grids, mags, n=20000, frac_transition=0.7, seed=1):
-    """Stratified choice: up to frac_transition * n grids with DN < m < UP,
-    the rest from outside the thresholds."""
+def select(grids, mags, n=20000, frac_transition=0.7, bins=18, seed=1):
+    """Stratified choice: frac_transition * n grids with DN < m < UP, spread
+    evenly over `bins` magnetisation bins (a bin short of grids gives its
+    share to the others), the rest from outside the thresholds."""
     rs = np.random.default_rng(seed)
     inside = np.nonzero((mags > DN) & (mags < UP))[0]
     outside = np.nonzero(~((mags > DN) & (mags < UP)))[0]
-    ni = min(inside.size, int(frac_transition * n))
-    no = min(outside.size, n - ni)
-    idx = np.concatenate([rs.choice(inside, ni, replace=False), rs.choice(outside, no, replace=False)])
+    want = min(inside.size, int(frac_transition * n))
+    edges = np.linspace(DN, UP, bins + 1)
+    members = [inside[(mags[inside] >= a) & (mags[inside] < b)] for a, b in zip(edges[:-1], edges[1:])]
+    take = np.zeros(bins, dtype=int)
+    while take.sum() < want:
+        open_ = [k for k in range(bins) if take[k] < members[k].size]
+        if not open_:
+            break
+        share = max(1, (want - take.sum()) // len(open_))
+        for k in open_:
+            take[k] = min(members[k].size, take[k] + share)
+            if take.sum() >= want:
+                break
+    chosen = [rs.choice(m, t, replace=False) for m, t in zip(members, take) if t > 0]
+    no = min(outside.size, n - int(take.sum()))
+    idx = np.concatenate(chosen + [rs.choice(outside, no, replace=False)])
     rs.shuffle(idx)
     return idx
 
 
-def label_by_shooting(grids, reps=256, seed=202, max_sweeps=20000, chk=100, chunk=4096, L=None):
+def label_by_shooting(grids, reps=256, seed=202, max_sweeps=20000, chk=100, chunk=4096, L=None, log=None):
     """pB of each grid by shooting: `reps` replicas per grid with independent
     Philox streams, until every fate is decided (or max_sweeps).  Returns
     (pB, nA, nB, seconds)."""
@@ -96,6 +134,8 @@ def label_by_shooting(grids, reps=256, seed=202, max_sweeps=20000, chk=100, chun
         st.free()
         nA[i0:i0 + k] = (f == 0).sum(1)
         nB[i0:i0 + k] = (f == 1).sum(1)
+        if log:
+            log(f"labelled {i0 + k}/{n} grids ({sw} sweeps in this chunk), {time.perf_counter() - t0:.1f} s")
     pB = nB / np.maximum(nA + nB, 1)
     return pB.astype(np.float32), nA, nB, time.perf_counter() - t0
 
@@ -154,9 +194,11 @@ def calibration(pred, label, bins=10):
     return out
 
 
-def estimate(grids, params, *, lo=0.2, hi=0.8, reps=1024, seed=303, max_sweeps=20000):
-    """Surrogate pB for all grids; shooting (reps replicas) for predictions in
-    [lo, hi].  Returns estimates, the fallback mask, and timings."""
+def estimate(grids, params, *, lo=0.2, hi=0.8, reps=128, seed=303, max_sweeps=20000):
+    """Surrogate pB for all grids; shooting (`reps` replicas each) for
+    predictions in [lo, hi].  Returns estimates, the fallback mask, and
+    timings.  128 replicas bound the shooting standard error by
+    sqrt(0.25 / 128) = 0.044."""
     L = grids.shape[1]
     t0 = time.perf_counter()
     net = Surrogate(L, 4096)
@@ -179,7 +221,8 @@ def main():
 
     ap = argparse.ArgumentParser(description="build the committor surrogate (data, labels, training)")
     ap.add_argument("--out", default="gpurun_out/surrogate")
-    ap.add_argument("--n", type=int, default=20000)
+    ap.add_argument("--n", type=int, default=14000, help="grids from nucleation trajectories")
+    ap.add_argument("--discs", type=int, default=6000, help="disc + noise grids (committor-input style)")
     ap.add_argument("--reps", type=int, default=256)
     ap.add_argument("--sweeps", type=int, default=50000)
     ap.add_argument("--epochs", type=int, default=40)
@@ -188,20 +231,37 @@ def main():
     init_device(-1)
     set_tuning(0, 0)
     t = time.perf_counter()
-    g, m, tt = sample_trajectories(sweeps=args.sweeps)
+    log = lambda msg: print(msg, flush=True)  # noqa: E731
+    g, m, tt = sample_trajectories(sweeps=args.sweeps, log=log)
     print(f"sampled {g.shape[0]} grids in {time.perf_counter() - t:.1f} s", flush=True)
     idx = select(g, m, args.n)
     g, m = g[idx], m[idx]
-    pB, nA, nB, ts = label_by_shooting(g, reps=args.reps)
+    if args.discs:
+        d = disc_grids(args.discs, 64, seed=777001).astype(np.int8)
+        g = np.concatenate([g, d])
+        m = np.concatenate([m, d.reshape(len(d), -1).mean(1, dtype=np.float64).astype(np.float32)])
+    kind = np.concatenate([np.zeros(len(idx), dtype=np.int8), np.ones(args.discs, dtype=np.int8)])
+    pB, nA, nB, ts = label_by_shooting(g, reps=args.reps, log=log)
     print(f"labelled {g.shape[0]} grids by shooting ({args.reps} replicas) in {ts:.1f} s", flush=True)
     np.savez_compressed(os.path.join(args.out, "dataset.npz"), grids=np.packbits(g.reshape(len(g), -1) > 0, axis=1),
-                        mag=m, pB=pB, nA=nA, nB=nB)
+                        mag=m, pB=pB, nA=nA, nB=nB, kind=kind)
     params, met = train(g, pB, epochs=args.epochs)
     np.save(os.path.join(args.out, "params.npy"), params)
     val = met["val_idx"]
-    summary = {"n": int(g.shape[0]), "reps": args.reps, "label_seconds": ts, "train_seconds": met["train_seconds"],
-               "val_mae": met["val_mae"], "calibration": calibration(met["val_pred"], pB[val]),
+    pv, yv, kv = met["val_pred"], pB[val], kind[val]
+    unsure = (pv >= 0.2) & (pv <= 0.8)
+    summary = {"n": int(g.shape[0]), "n_trajectory": int(len(idx)), "n_discs": int(args.discs), "reps": args.reps,
+               "label_seconds": ts, "train_seconds": met["train_seconds"], "epochs": args.epochs,
+               "val_n": int(len(val)), "val_mae": met["val_mae"],
+               "val_mae_trajectory": float(np.mean(np.abs(pv - yv)[kv == 0])),
+               "val_mae_discs": float(np.mean(np.abs(pv - yv)[kv == 1])),
+               "val_fallback_fraction": float(unsure.mean()),
+               "val_mae_outside_fallback": float(np.mean(np.abs(pv - yv)[~unsure])),
+               "calibration": calibration(pv, yv),
                "loss_history": [float(x) for x in met["loss_history"]]}
+    np.save(os.path.join(args.out, "val_idx.npy"), val)
+    np.savez(os.path.join(args.out, "surrogate_L64.npz"), params=params, val_mae=summary["val_mae"],
+             n_train=len(g) - len(val), epochs=args.epochs, reps=args.reps)
     with open(os.path.join(args.out, "summary.json"), "w") as f:
         json.dump(summary, f, indent=1)
     print(json.dumps({k: v for k, v in summary.items() if k != "loss_history"}), flush=True)

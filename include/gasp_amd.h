@@ -173,7 +173,10 @@ void gasp_free(void *d_ptr);
 
 /* Engine tuning: slots per replica (number of lanes that speculate on
  * consecutive attempts of one replica; 0 = automatic) and waves per workgroup
- * for methods 0/1 (0 = automatic).  Applies to subsequent runs. */
+ * for methods 0/1 (0 = automatic).  For method 2 the second value is the word
+ * split: 1 = a workgroup holds the 64 replicas of a 64-bit site word (slots up
+ * to 16), 2 = it holds one 32-bit half (32 replicas, slots up to 32).
+ * Applies to subsequent runs. */
 void gasp_set_tuning(int slots_per_replica, int waves_per_group);
 void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
                      int *waves_per_group);
@@ -209,6 +212,10 @@ int gasp_engine_set_inputs(gasp_engine *e, const int *inputs, int ninputs);
  * sweeping them). */
 int gasp_engine_set_freeze(gasp_engine *e, int freeze);
 int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device);
+/* Grids of the replicas listed in replicas[0..n) as int8 +-1, [n][L*L], in
+ * host memory: the int8 snapshot copies of the Python module's grid history
+ * (src/ising_ext.cu:280-463) without moving every replica's int32 grid. */
+int gasp_engine_gather_grids(gasp_engine *e, const int *replicas, int n, int8_t *host_out);
 /* Advance every replica by nsweeps sweeps.  check: 0 none, 1 count replicas
  * with m > up_thr, 2 update fates (sticky; 1 if m > up_thr, 0 if
  * m < dn_thr).  The magnetisation after the sweeps is always computed. */

@@ -35,7 +35,7 @@ def build(with_ref: bool | None = None) -> None:
     if with_ref is None:
         with_ref = os.path.isdir(os.path.join(REFERENCE_ROOT, "src"))
     if with_ref:
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "ref-io"], check=True)
 
 
 _lib = None

@@ -1,6 +1,7 @@
 """Bit-exact parity of the HIP sweep kernels with the CPU oracle.
 
-Every lattice layout (gpu_method 0/1/2) and every speculation width S is run
+Every lattice layout (gpu_method 0/1/2, method 2 with whole or split 64-bit
+site words) and every speculation width S is run
 through the C ABI (gasp_engine_*) and compared spin for spin with
   - oracle.sweep_mt_per_replica: src/mc_cpu.c:41-89 restated, per-replica
     MT19937 streams seeded seed + r (the reference CPU stream), and
@@ -15,7 +16,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 BETA, H = 0.54, 0.07
-CASES = [(2, s) for s in (1, 2, 4, 8, 16)] + [(1, s) for s in (1, 2, 8, 64)] + [(0, s) for s in (1, 4, 64)]
+# (method, slots per replica S, second tuning value: waves for 0/1, word split for 2)
+CASES = ([(2, s, 1) for s in (1, 2, 4, 8, 16)] + [(2, s, 2) for s in (1, 4, 16, 32)] +
+         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)])
 
 
 def _start(kind, R, L, seed):
@@ -25,8 +28,8 @@ def _start(kind, R, L, seed):
     return np.where(rs.random((R, L, L)) < 0.5, 1, -1).astype(np.int32)
 
 
-def _run_engine(G, L, R, method, S, g0, seed, mt, beta, h, chunks):
-    G.set_tuning(S, 1)
+def _run_engine(G, L, R, method, S, g0, seed, mt, beta, h, chunks, W=1):
+    G.set_tuning(S, W)
     try:
         rng = G.RngState(R, seed, mt=mt)
         with G.Engine(L, R, method, rng, beta, h) as e:
@@ -41,14 +44,14 @@ def _run_engine(G, L, R, method, S, g0, seed, mt, beta, h, chunks):
     return out, mag
 
 
-@pytest.mark.parametrize("method,S", CASES)
+@pytest.mark.parametrize("method,S,W", CASES)
 @pytest.mark.parametrize("L", [16, 17])
-def test_mt_stream_bit_exact(gpu, engine_lib, oracle, method, S, L):
+def test_mt_stream_bit_exact(gpu, engine_lib, oracle, method, S, W, L):
     G, O = engine_lib, oracle
     R, seed, beta = 96, 1234567, 0.3
-    print(f"[mt] method={method} S={S} L={L} R={R}", flush=True)
+    print(f"[mt] method={method} S={S} W={W} L={L} R={R}", flush=True)
     g0 = _start("random", R, L, seed)
-    got, mag = _run_engine(G, L, R, method, S, g0, seed, True, beta, 0.07, [3, 2])
+    got, mag = _run_engine(G, L, R, method, S, g0, seed, True, beta, 0.07, [3, 2], W)
     ref = O.sweep_mt_per_replica(L, g0.copy(), seed, 5, beta, 0.07)
     assert np.array_equal(got, ref)
     ref_m = np.array([O.lib().orc_magnetisation_gpu(L, ref[r].reshape(-1)) for r in range(R)],
@@ -56,15 +59,15 @@ def test_mt_stream_bit_exact(gpu, engine_lib, oracle, method, S, L):
     assert np.array_equal(mag, ref_m)
 
 
-@pytest.mark.parametrize("method,S", CASES)
+@pytest.mark.parametrize("method,S,W", CASES)
 @pytest.mark.parametrize("start", ["random", "down"])
-def test_philox_stream_bit_exact(gpu, engine_lib, oracle, method, S, start):
+def test_philox_stream_bit_exact(gpu, engine_lib, oracle, method, S, W, start):
     G, O = engine_lib, oracle
     L, R, seed = 16, 100, 987654321
     beta = 0.3 if start == "random" else BETA
-    print(f"[philox] method={method} S={S} start={start}", flush=True)
+    print(f"[philox] method={method} S={S} W={W} start={start}", flush=True)
     g0 = _start(start, R, L, seed)
-    got, mag = _run_engine(G, L, R, method, S, g0, seed, False, beta, H, [4, 3])
+    got, mag = _run_engine(G, L, R, method, S, g0, seed, False, beta, H, [4, 3], W)
     ref = O.sweep_philox(L, g0.copy(), seed, 0, 7 * L * L, beta, H)
     assert np.array_equal(got, ref)
     ref_m = (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32)
@@ -81,6 +84,19 @@ def test_methods_agree_at_L64(gpu, engine_lib, oracle, method):
     got, _ = _run_engine(G, L, R, method, 0, g0, seed, False, BETA, H, [10])
     ref = O.sweep_philox(L, g0.copy(), seed, 0, 10 * L * L, BETA, H)
     assert np.array_equal(got, ref)
+
+
+def test_split_words_at_L64(gpu, engine_lib, oracle):
+    """Split 64-bit words (32 replicas per workgroup) at the widest
+    speculation, L = 64, with a partial last group (R = 100)."""
+    G, O = engine_lib, oracle
+    L, R, seed = 64, 100, 2894203475
+    print("[L64 split] S=32", flush=True)
+    g0 = _start("down", R, L, seed)
+    got, mag = _run_engine(G, L, R, 2, 32, g0, seed, False, BETA, H, [6, 4], W=2)
+    ref = O.sweep_philox(L, g0.copy(), seed, 0, 10 * L * L, BETA, H)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
 @pytest.mark.parametrize("L", [8, 16, 33, 64])
@@ -103,8 +119,8 @@ def test_largest_cluster(gpu, engine_lib, oracle, L):
     assert np.array_equal(got_dn, ref_dn)
 
 
-@pytest.mark.parametrize("method", [1, 2])
-def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method):
+@pytest.mark.parametrize("method,W", [(1, 0), (2, 0), (2, 2)])
+def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method, W):
     """gasp_engine_set_inputs replicates input grids exactly as uploading the
     replicated grids; freezing decided replicas changes no fate at any check,
     and no undecided replica's trajectory."""
@@ -113,8 +129,9 @@ def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method):
     R = nin * per
     inputs = np.stack([np.where(np.add.outer(np.arange(L), np.arange(L)) % (k + 3) == 0, 1, -1)
                        for k in range(nin)]).astype(np.int32)
-    print(f"[freeze] method={method}", flush=True)
+    print(f"[freeze] method={method} W={W}", flush=True)
     runs = []
+    G.set_tuning(0, W)
     for freeze in (False, True):
         rng = G.RngState(R, 99)
         with G.Engine(L, R, method, rng, 0.54, 0.07) as e:
@@ -133,8 +150,31 @@ def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method):
                 hist.append((e.counts().copy(), f, e.get_grids()))
         rng.free()
         runs.append(hist)
+    G.set_tuning(0, 0)
     for (c0, f0, g0), (c1, f1, g1) in zip(*runs):
         assert np.array_equal(c0[1:], c1[1:]) and np.array_equal(f0, f1)
         und = f1 == -1
         assert np.array_equal(g0[und], g1[und])
     assert (runs[1][-1][1] != -1).sum() > 0  # some replicas did freeze
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_gather_grids_int8(gpu, engine_lib, method):
+    """gasp_engine_gather_grids returns exactly the selected rows of the full
+    int32 grids, as int8, for any order and repeats of replica indices."""
+    G = engine_lib
+    L, R = 24, 130
+    print(f"[gather] method={method}", flush=True)
+    g0 = _start("random", R, L, 5)
+    rng = G.RngState(R, 5)
+    with G.Engine(L, R, method, rng, 0.3, 0.0) as e:
+        e.set_grids(g0)
+        e.sweep(3)
+        full = e.get_grids()
+        idx = np.array([129, 0, 64, 63, 7, 7, 128, 65], dtype=np.int32)
+        got = e.gather(idx)
+        with pytest.raises(G.GaspError):
+            e.gather([R])
+    rng.free()
+    assert got.dtype == np.int8
+    assert np.array_equal(got, full[idx].astype(np.int8))

@@ -86,6 +86,33 @@ def _import_gasp(engine_lib):
     return gasp
 
 
+class _H5:
+    """Reads int datasets through the HDF5 C library (hid_t is int64 in 1.10)."""
+
+    def __init__(self, path="/opt/conda/lib/libhdf5.so"):
+        self.lib = C.CDLL(path)
+        self.lib.H5open()
+        for f in ("H5Fopen", "H5Dopen2"):
+            getattr(self.lib, f).restype = C.c_int64
+        self.lib.H5Fopen.argtypes = [C.c_char_p, C.c_uint, C.c_int64]
+        self.lib.H5Dopen2.argtypes = [C.c_int64, C.c_char_p, C.c_int64]
+        self.lib.H5Dread.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+        self.lib.H5Dclose.argtypes = [C.c_int64]
+        self.lib.H5Fclose.argtypes = [C.c_int64]
+        self.native_int = C.c_int64.in_dll(self.lib, "H5T_NATIVE_INT_g").value
+
+    def read_int(self, path, name, shape):
+        f = self.lib.H5Fopen(path.encode(), 0, 0)  # H5F_ACC_RDONLY, H5P_DEFAULT
+        assert f >= 0, path
+        d = self.lib.H5Dopen2(f, name.encode(), 0)
+        assert d >= 0, name
+        out = np.zeros(shape, dtype=np.int32)
+        assert self.lib.H5Dread(d, self.native_int, 0, 0, 0, out.ctypes.data) >= 0  # H5S_ALL = 0
+        self.lib.H5Dclose(d)
+        self.lib.H5Fclose(f)
+        return int(out) if shape == () else out
+
+
 def test_python_swarm(gpu, engine_lib, oracle, tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     gasp = _import_gasp(engine_lib)
@@ -114,10 +141,21 @@ def test_python_swarm(gpu, engine_lib, oracle, tmp_path, monkeypatch):
     for s, (sw, i, m, grid) in zip(flat, exp):
         assert (s.isweep, s.igrid) == (sw, i) and s.magnetisation == np.float32(m) and s.committor == -1.0
         assert s.grid.dtype == np.int8 and np.array_equal(s.grid, grid)
-    # history limit
+    # HDF5 grid file (src/io.c layout): every replica at every dump
+    h5 = _H5()
+    dumps = list(ref["dump_sweeps"])
+    assert h5.read_int("gridstates.hdf5", "/total_saved_grids", ()) == R * len(dumps)
+    assert h5.read_int("gridstates.hdf5", "/L", ()) == L
+    for k in (0, len(dumps) - 1):
+        for i in (0, R - 1):
+            g = h5.read_int("gridstates.hdf5", f"/sweep_{int(dumps[k])}/grid_{i}", (L, L))
+            assert np.array_equal(g, ref["dumps"][k][i].reshape(L, L))
+    # history limit; no HDF5 file on request
+    os.remove("gridstates.hdf5")
     gasp.run_nucleation_swarm(L, R, 300, BETA, H, mag_output_int=20, grid_output_int=40, seed=seed,
-                              rng="mt19937", max_keep_grids=5)
+                              rng="mt19937", max_keep_grids=5, write_hdf5=False)
     assert sum(len(s) for s in gasp.grids) == 5
+    assert not os.path.exists("gridstates.hdf5")
 
 
 def test_python_committor(gpu, engine_lib, oracle, tmp_path, monkeypatch):
