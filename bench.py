@@ -36,6 +36,7 @@ UP_THR, DN_THR = 0.90, -0.90
 # bank word holding the spin bit per attempt; peak = 256 CU x 128 B/clk x 2.4 GHz.
 LDS_BYTES_PER_ATTEMPT = 20
 LDS_PEAK_GBS = 256 * 128 * 2.4
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 METRIC = "spin-flip attempts/sec (1 GPU, L=64) + committor pB estimates/sec at MAE≤0.05"
 
 
@@ -85,6 +86,9 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
     errs = [lib.bootstrap_errbar(per, per, np.ascontiguousarray(f[i]).ctypes.data_as(C.POINTER(C.c_int)))
             for i in range(n_inputs)]
     t_boot = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    errs_gpu = G.bootstrap_gpu(f, per, seed=1)
+    t_boot_gpu = time.perf_counter() - t1
     return {
         "metric": "committor pB estimates/sec (shooting)",
         "value": n_inputs / (t_gpu + t_boot),
@@ -93,6 +97,9 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
         "inputs": n_inputs, "replicas_per_input": per, "sweeps": sweeps,
         "unresolved": int(R_c - c[1] - c[2]), "gpu_seconds": t_gpu, "kernel_seconds": kern_ms / 1e3,
         "bootstrap_seconds": t_boot,
+        "value_gpu_bootstrap": n_inputs / (t_gpu + t_boot_gpu),
+        "bootstrap_gpu_seconds": t_boot_gpu,
+        "mean_bootstrap_error_gpu": float(np.nanmean(errs_gpu)),
         "pB_quantiles": [float(x) for x in np.nanquantile(pB, [0.0, 0.25, 0.5, 0.75, 1.0])],
         "mean_bootstrap_error": float(np.mean(errs)),
         "workload": "512 grids: -1 sea, +1 disc of radius U[4,14] at a random centre, 5% of spins "
@@ -221,13 +228,13 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
     except AttributeError:
         ncores = os.cpu_count() or 1
     ncores = max(1, min(ncores, 16))
-    nrep = 4 * ncores
+    nrep = 8 * ncores
     g = -np.ones((nrep, L, L), dtype=np.int32)
     # calibrate with one sweep, then size the sample to ~max_seconds
     t0 = time.perf_counter()
     O.sweep_mt_per_replica(L, g, 2894203475, 1, BETA, H, nthreads=ncores)
     dt = time.perf_counter() - t0
-    nsweeps = int(max(1, min(2000, max_seconds / max(dt, 1e-6))))
+    nsweeps = int(max(1, min(200000, max_seconds / max(dt, 1e-6))))
     g = -np.ones((nrep, L, L), dtype=np.int32)
     t0 = time.perf_counter()
     O.sweep_mt_per_replica(L, g, 2894203475, nsweeps, BETA, H, nthreads=ncores)
@@ -339,11 +346,16 @@ def main() -> int:
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic_sweep_multispin_L64.json")
     if os.path.exists(traffic_file):
-        try:
-            with open(traffic_file) as f:
-                out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
-        except Exception:
-            pass
+        with open(traffic_file) as f:
+            prof = json.load(f)
+        out["roofline"]["traffic"] = prof.get("hbm_bytes_per_launch")
+        vpa = prof.get("valu_instructions_per_attempt")
+        if vpa:
+            # VALU issue: wave64 instructions per attempt (rocprofv3 SQ_INSTS_VALU over the same
+            # launch) x the live attempt rate, against 256 CU x 4 SIMD x 2.4 GHz / 4 cycles
+            out["roofline"]["valu"] = {"instructions_per_attempt": vpa, "achieved": kern_rate * vpa,
+                                       "peak": VALU_PEAK, "unit": "wave64 instr/s",
+                                       "frac": kern_rate * vpa / VALU_PEAK}
     G.set_tuning(0, 0)
     if not args.no_committor:
         out["committor"] = committor_leg(G)

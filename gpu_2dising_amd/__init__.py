@@ -35,6 +35,7 @@ EXPORTS = (
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
     "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
+    "gasp_bootstrap_errbar_gpu",
     "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
     "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
     "gasp_cnn_set_data", "gasp_cnn_train_epoch",
@@ -162,6 +163,8 @@ def lib() -> C.CDLL:
     L.gasp_graph_destroy.restype = C.c_int
     L.gasp_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp]
     L.gasp_autotune.restype = C.c_int
+    L.gasp_bootstrap_errbar_gpu.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, vp]
+    L.gasp_bootstrap_errbar_gpu.restype = C.c_int
     i8p = C.POINTER(C.c_int8)
     for name in ("gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
                  "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
@@ -400,6 +403,16 @@ def autotune(method: int, L: int, ngrids: int, probe_sweeps: int = 5):
     s, w, r = C.c_int(), C.c_int(), C.c_double()
     check(lib().gasp_autotune(method, L, ngrids, probe_sweeps, C.byref(s), C.byref(w), C.byref(r)))
     return s.value, w.value, r.value
+
+
+def bootstrap_gpu(fates: np.ndarray, m: int, seed: int = 1) -> np.ndarray:
+    """GPU bootstrap error bars of pB for fates [ninputs, n] (-1/0/1)."""
+    f = np.ascontiguousarray(fates, dtype=np.int32)
+    if f.ndim != 2:
+        raise ValueError("fates must be [ninputs, n]")
+    out = np.empty(f.shape[0], dtype=np.float64)
+    check(lib().gasp_bootstrap_errbar_gpu(f.ctypes.data, 0, f.shape[0], f.shape[1], m, seed, out.ctypes.data))
+    return out
 
 
 def set_tuning(slots: int = 0, waves: int = 0) -> None:

@@ -893,6 +893,107 @@ extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
 
 extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
 
+// ============================================================ GPU bootstrap
+// bootstrap_errbar (src/bootstrap.c:6-63) for many inputs at once, with
+// counter-based draws in place of the process-global libc rand() sequence:
+// the same estimator (m resamples of n indices with replacement, pB* =
+// nB / (nA + nB) over resolved fates, population standard deviation of pB*),
+// a different random sequence.  One workgroup per input, its fates in LDS as
+// bytes (1 = A, 2 = B, 0 = unresolved); thread t takes resamples t, t + 256,
+// ...; index draw k of resample j is word k % 4 of the Philox4x32-10 block
+// ctr = (k / 4, j, input, 0x424f4f54), key = seed, mapped as floor(w n / 2^32).
+// Per-thread Welford accumulators are combined across the block (Chan et al.).
+constexpr uint32_t kBootDomain = 0x424f4f54u;  // "BOOT"
+
+__global__ void __launch_bounds__(256) bootstrap_kernel(const int *__restrict__ fates, int n, int m, uint64_t seed,
+                                                        double *__restrict__ out) {
+    extern __shared__ __align__(16) unsigned char f[];
+    __shared__ double s_cnt[256], s_mean[256], s_m2[256];
+    const int tid = threadIdx.x, input = blockIdx.x;
+    const int *src = fates + (size_t)input * n;
+    for (int i = tid; i < n; i += blockDim.x) f[i] = src[i] == 0 ? 1 : (src[i] == 1 ? 2 : 0);
+    __syncthreads();
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    double cnt = 0.0, mean = 0.0, m2 = 0.0;
+    for (int j = tid; j < m; j += blockDim.x) {
+        int nA = 0, nB = 0;
+        for (int i = 0; i < n; i += 4) {
+            uint32_t c0 = (uint32_t)(i >> 2), c1 = (uint32_t)j, c2 = (uint32_t)input, c3 = kBootDomain;
+            philox4x32_10(c0, c1, c2, c3, k0, k1);
+            const uint32_t w[4] = {c0, c1, c2, c3};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (i + q < n) {
+                    const unsigned char v = f[((uint64_t)w[q] * (uint32_t)n) >> 32];
+                    nA += v == 1;
+                    nB += v == 2;
+                }
+            }
+        }
+        const double p = (double)nB / (double)(nA + nB);  // NaN without resolved draws, as the reference
+        cnt += 1.0;
+        const double d = p - mean;
+        mean += d / cnt;
+        m2 += d * (p - mean);
+    }
+    s_cnt[tid] = cnt;
+    s_mean[tid] = mean;
+    s_m2[tid] = m2;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            const double na = s_cnt[tid], nb = s_cnt[tid + off];
+            const double nt = na + nb;
+            if (nb > 0.0) {
+                const double d = s_mean[tid + off] - s_mean[tid];
+                s_mean[tid] = na > 0.0 ? s_mean[tid] + d * nb / nt : s_mean[tid + off];
+                s_m2[tid] = s_m2[tid] + s_m2[tid + off] + (na > 0.0 ? d * d * na * nb / nt : 0.0);
+                s_cnt[tid] = nt;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) out[input] = sqrt(s_m2[0] / (double)m);
+}
+
+extern "C" int gasp_bootstrap_errbar_gpu(const int *fates, int on_device, int ninputs, int n, int m,
+                                         unsigned long long seed, double *host_out) {
+    clear_error();
+    if (!fates || !host_out || ninputs < 1 || n < 1 || m < 1)
+        return set_error(GASP_ERR_ARG, "gasp_bootstrap_errbar_gpu: bad argument");
+    DeviceInfo di = device_info();
+    if (di.ok && (size_t)n > di.lds_block - 3 * 256 * sizeof(double))
+        return set_error(GASP_ERR_ARG, "gasp_bootstrap_errbar_gpu: %d fates per input exceed the LDS", n);
+    const int *d_f = fates;
+    int *d_own = nullptr;
+    double *d_out = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t he = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (he == hipSuccess && !on_device) {
+        he = hipMalloc(&d_own, (size_t)ninputs * n * sizeof(int));
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(d_own, fates, (size_t)ninputs * n * sizeof(int), hipMemcpyHostToDevice, st);
+        d_f = d_own;
+    }
+    if (he == hipSuccess) he = hipMalloc(&d_out, (size_t)ninputs * sizeof(double));
+    if (he == hipSuccess && (size_t)n + 3 * 256 * sizeof(double) > 65536)
+        he = hipFuncSetAttribute((const void *)bootstrap_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, n);
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(bootstrap_kernel, dim3(ninputs), dim3(256), (size_t)n, st, d_f, n, m, (uint64_t)seed,
+                           d_out);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(host_out, d_out, (size_t)ninputs * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    dfree(d_own);
+    dfree(d_out);
+    if (st) (void)hipStreamDestroy(st);
+    if (he != hipSuccess)
+        return set_error(he == hipErrorOutOfMemory ? GASP_ERR_NOMEM : GASP_ERR_HIP, "GPUassert: %s %s %d",
+                         hipGetErrorString(he), __FILE__, __LINE__);
+    return GASP_OK;
+}
+
 // ============================================================ batched graph
 // One captured hipGraph advancing several engines (different L, method,
 // temperature, task) concurrently: the capture stream forks to every engine's

@@ -207,12 +207,18 @@ __device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stenc
 }
 
 // ------------------------------------------------------------ conflict filter
-// Bits of the rows (or columns) within periodic distance 1 of t, folded mod
-// 64.  Exact for L <= 64; for power-of-two L > 64, entry (r & 63) holds the
-// fold of the rows around r (64 divides L, so (r +- 1 mod L) & 63 =
-// ((r & 63) +- 1) & 63).  An attempt at (row, col) can only conflict with an
-// accepted attempt whose row mask holds `row` and whose column mask holds
-// `col`.
+// An attempt at (r', c') conflicts with an accepted attempt at (r, c) iff
+// (dr, dc) = (r' - r, c' - c) mod L is one of the five stencil offsets.  Then
+// each of the projections r, c, r + c and r - c (mod L) differs by at most 1;
+// conversely, if all four differ by at most 1, (dr, dc) is a stencil offset
+// (the diagonal offsets fail r + c or r - c).  The filter tests the four
+// projections of each attempt against the unions, over the earlier accepted
+// attempts of its group, of their "within 1" sets: exact for one earlier
+// accepted attempt, conservative for several.
+// nbr_mask(t): bits of the values within periodic distance 1 of t, folded
+// mod 64.  Exact for L <= 64; for power-of-two L > 64, entry (t & 63) holds
+// the fold of the values around t (64 divides L, so (t +- 1 mod L) & 63 =
+// ((t & 63) +- 1) & 63).
 __device__ __forceinline__ uint64_t nbr_mask(int t, int L) {
     int rm1, rp1;
     if (L <= 64) {
@@ -232,33 +238,38 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + K, 0xf, 0xf, true);
 }
 
-template <int S, int K>
-__device__ __forceinline__ void scan_step(uint32_t &lo, uint32_t &hi, int slot) {
-    uint32_t tl = row_shr<K>(lo), th = row_shr<K>(hi);
-    if constexpr (S < 16) {
-        // the source lane belongs to the previous group
-        if (slot < K) tl = th = 0u;
+// OR of w[i] over the lanes of the group (S <= 16 consecutive, aligned lanes)
+// that precede this lane, for each of the NW dwords: a shift by one lane, then
+// an inclusive Hillis-Steele OR-scan (steps k < S) inside the 16-lane DPP row.
+template <int S, int K, int NW>
+__device__ __forceinline__ void scan_step(uint32_t (&w)[NW], int slot) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        uint32_t t = row_shr<K>(w[i]);
+        if constexpr (S < 16) {
+            if (slot < K) t = 0u;  // the source lane belongs to the previous group
+        }
+        w[i] |= t;
     }
-    lo |= tl;
-    hi |= th;
 }
 
-// OR of x over the lanes of the group (S <= 16 consecutive lanes, aligned)
-// that precede this lane.
-template <int S>
-__device__ __forceinline__ uint64_t group_excl_or(uint64_t x, int slot) {
+template <int S, int NW>
+__device__ __forceinline__ void group_excl_or(uint32_t (&w)[NW], int slot) {
     static_assert(S >= 2 && S <= 16, "DPP scan within a 16-lane row");
-    uint32_t lo = row_shr<1>((uint32_t)x), hi = row_shr<1>((uint32_t)(x >> 32));
-    if constexpr (S < 16) {
-        if (slot == 0) lo = hi = 0u;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        w[i] = row_shr<1>(w[i]);
+        if constexpr (S < 16) {
+            if (slot == 0) w[i] = 0u;
+        }
     }
-    // inclusive OR-scan of the shifted values (Hillis-Steele: steps k < S)
-    scan_step<S, 1>(lo, hi, slot);
-    if constexpr (S > 2) scan_step<S, 2>(lo, hi, slot);
-    if constexpr (S > 4) scan_step<S, 4>(lo, hi, slot);
-    if constexpr (S > 8) scan_step<S, 8>(lo, hi, slot);
-    return ((uint64_t)hi << 32) | lo;
+    scan_step<S, 1>(w, slot);
+    if constexpr (S > 2) scan_step<S, 2>(w, slot);
+    if constexpr (S > 4) scan_step<S, 4>(w, slot);
+    if constexpr (S > 8) scan_step<S, 8>(w, slot);
 }
+
+__device__ __forceinline__ uint32_t fold32(uint64_t m) { return (uint32_t)m | (uint32_t)(m >> 32); }
 
 // Per-replica epilogue: magnetisation and the threshold check.  Must be
 // called by a whole wave (ballots); `own` marks the lane reporting replica r.
@@ -430,19 +441,39 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             uint64_t cA = 0ull, cB = __ballot(ownB);
             uint64_t m = mA | mB;
             if constexpr (S <= 16) {
-                // Filter: OR the row / column neighbourhood masks of the
-                // accepted attempts of the lanes before this one in the
-                // group; if no attempt of the wave has both its row and its
-                // column covered, no attempt conflicts and the exact loop is
-                // skipped (it runs unchanged otherwise).
+                // Filter (see nbr_mask): union the projection masks of the
+                // accepted attempts of the earlier lanes of the group; if no
+                // attempt of the wave lies within 1 on all four projections,
+                // none conflicts and the exact loop is skipped (it runs
+                // unchanged otherwise).  Rows and columns use 64-bit masks,
+                // the two diagonals masks folded to 32 bits.
                 if (use_filter && m != 0ull) {
-                    const uint64_t rA = nbr_lds[A.st.row & 63], kA = nbr_lds[A.st.col & 63];
-                    const uint64_t rB = nbr_lds[B.st.row & 63], kB = nbr_lds[B.st.col & 63];
-                    const uint64_t rm = (A.acc ? rA : 0ull) | (B.acc ? rB : 0ull);
-                    const uint64_t km = (A.acc ? kA : 0ull) | (B.acc ? kB : 0ull);
-                    const uint64_t er = group_excl_or<S>(rm, slot), ek = group_excl_or<S>(km, slot);
-                    const bool sA = ((er >> (A.st.row & 63)) & (ek >> (A.st.col & 63)) & 1ull) != 0ull;
-                    const bool sB = ((er >> (B.st.row & 63)) & (ek >> (B.st.col & 63)) & 1ull) != 0ull;
+                    int pA[4], pB[4];  // row, col, (row + col) mod L, (row - col) mod L
+                    pA[0] = A.st.row, pA[1] = A.st.col, pB[0] = B.st.row, pB[1] = B.st.col;
+                    if constexpr (POW2) {
+                        pA[2] = (pA[0] + pA[1]) & (L - 1), pA[3] = (pA[0] - pA[1]) & (L - 1);
+                        pB[2] = (pB[0] + pB[1]) & (L - 1), pB[3] = (pB[0] - pB[1]) & (L - 1);
+                    } else {
+                        pA[2] = pA[0] + pA[1], pA[3] = pA[0] - pA[1];
+                        pB[2] = pB[0] + pB[1], pB[3] = pB[0] - pB[1];
+                        pA[2] -= pA[2] >= L ? L : 0, pA[3] += pA[3] < 0 ? L : 0;
+                        pB[2] -= pB[2] >= L ? L : 0, pB[3] += pB[3] < 0 ? L : 0;
+                    }
+                    uint64_t mAk[4], mBk[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        mAk[k] = A.acc ? nbr_lds[pA[k] & 63] : 0ull;
+                        mBk[k] = B.acc ? nbr_lds[pB[k] & 63] : 0ull;
+                    }
+                    const uint64_t rm = mAk[0] | mBk[0], km = mAk[1] | mBk[1];
+                    uint32_t w[6] = {(uint32_t)rm, (uint32_t)(rm >> 32), (uint32_t)km, (uint32_t)(km >> 32),
+                                     fold32(mAk[2] | mBk[2]), fold32(mAk[3] | mBk[3])};
+                    group_excl_or<S>(w, slot);
+                    const uint64_t er = ((uint64_t)w[1] << 32) | w[0], ek = ((uint64_t)w[3] << 32) | w[2];
+                    const bool sA = ((er >> (pA[0] & 63)) & (ek >> (pA[1] & 63)) & 1ull) &&
+                                    ((w[4] >> (pA[2] & 31)) & (w[5] >> (pA[3] & 31)) & 1u);
+                    const bool sB = ((er >> (pB[0] & 63)) & (ek >> (pB[1] & 63)) & 1ull) &&
+                                    ((w[4] >> (pB[2] & 31)) & (w[5] >> (pB[3] & 31)) & 1u);
                     if (__ballot(sA || sB) == 0ull) m = 0ull;
                 }
             }

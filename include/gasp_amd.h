@@ -248,6 +248,14 @@ double gasp_graph_ms(gasp_graph *g);               /* GPU time of all replays */
 int gasp_graph_destroy(gasp_graph *g);
 /* Times the candidate launch shapes (slots per replica, waves per workgroup)
  * for (method, L, ngrids) and keeps the fastest for later engines. */
+/* bootstrap_errbar (src/bootstrap.c:6-63) for `ninputs` inputs of n fates each
+ * (int32 [ninputs][n], -1/0/1; host memory, or device memory if on_device)
+ * on the GPU: m resamples per input, the same estimator with Philox4x32-10
+ * draws (key `seed`) instead of the process-global libc rand() sequence, so
+ * the error bars agree statistically, not bit for bit (the default
+ * bootstrap_errbar keeps the reference's sequence).  host_out[ninputs]. */
+int gasp_bootstrap_errbar_gpu(const int *fates, int on_device, int ninputs, int n, int m,
+                              unsigned long long seed, double *host_out);
 int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
                   double *best_rate);
 

@@ -179,5 +179,19 @@ def test_python_committor(gpu, engine_lib, oracle, tmp_path, monkeypatch):
     errs = [oracle.lib().orc_bootstrap_errbar(R // 2, R // 2, np.ascontiguousarray(ref["fates"][k * R // 2:(k + 1) * R // 2]))
             for k in range(2)]
     assert [e for _, e in res] == errs
+    # the same run with GPU bootstrap error bars: same pB, statistically equal errors
+    # (64 resamples of 64 fates each: a few tens of percent of noise per estimate)
+    res_g = gasp.run_committor_calc(L, R, 3000, BETA, H, grid_input="NumPy", grid_array=inputs, nsms=2,
+                                    mag_output_int=10, grid_output_int=100, seed=seed, rng="mt19937",
+                                    up_threshold=0.9, dn_threshold=-0.93, bootstrap="gpu", write_hdf5=False)
+    assert [p for p, _ in res_g] == [p for p, _ in res]
+    for (_, eg), (p, e) in zip(res_g, res):
+        if 0.0 < p < 1.0:
+            assert 0.5 * e < eg < 2.0 * e
+        else:
+            assert eg == 0.0 and e == 0.0
     with pytest.raises(ValueError):
         gasp.run_committor_calc(L, 102, 10, BETA, H, grid_input="NumPy", grid_array=inputs, nsms=4)
+    with pytest.raises(ValueError):
+        gasp.run_committor_calc(L, R, 10, BETA, H, grid_input="NumPy", grid_array=inputs, nsms=2,
+                                bootstrap="numpy")
