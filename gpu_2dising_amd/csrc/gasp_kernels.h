@@ -71,8 +71,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32
     for (int i = 0; i < 10; ++i) {
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // three-input XOR in one v_bitop3_b32 (truth table 0x96 = parity)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -200,8 +201,10 @@ __device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stenc
     Attempt t;
     t.st = st;
     t.s01 = lat.spin(st.c);
-    const int nup = lat.spin(st.up) + lat.spin(st.dn) + lat.spin(st.lt) + lat.spin(st.rt);
-    const uint64_t T = thr_lds[5 * t.s01 + nup];
+    const unsigned nup = (unsigned)(lat.spin(st.up) + lat.spin(st.dn)) + (unsigned)(lat.spin(st.lt) + lat.spin(st.rt));
+    // byte offset of entry 5 s01 + nup: 40 s01 + 8 nup
+    const unsigned off = (unsigned)t.s01 * 40u + (nup << 3);
+    const uint64_t T = *(const uint64_t *)((const char *)thr_lds + off);
     t.acc = en && ((uint64_t)ua < T);
     return t;
 }
