@@ -6,7 +6,8 @@ h = 0.07, 16384 replicas (256 words of 64 replicas, gpu_method 2), all spins
 -1 at start, Philox4x32-10 streams.  One step = one 100-sweep launch of the
 sweep kernel with the fused magnetisation / nucleation check, the host reading
 the nucleated count (the driver's mag_output_int cadence); every 10th step
-also dumps the grids (grid_output_int = 1000) as int32 to host memory.  The
+also copies the grids (grid_output_int = 1000) to host memory as the body of a
+bit-packed gridstates.bin record (8.4 MB, packed on the device).  The
 default 500 steps are the full 50000-sweep run; the nucleated fraction curve
 is fitted to 1 - exp(-k (t - t0)).
 
@@ -35,6 +36,8 @@ UP_THR, DN_THR = 0.90, -0.90
 # Roofline of the LDS-resident multispin kernel: 5 reads of the 4-byte LDS
 # bank word holding the spin bit per attempt; peak = 256 CU x 128 B/clk x 2.4 GHz.
 LDS_BYTES_PER_ATTEMPT = 20
+HBM_BYTES_PER_ATTEMPT = 5
+HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4
 VALU_PEAK = 256 * 4 * 2.4e9 / 4
 METRIC = "spin-flip attempts/sec (1 GPU, L=64) + committor pB estimates/sec at MAE≤0.05"
@@ -219,30 +222,49 @@ def fit_rate(sweeps: np.ndarray, frac: np.ndarray):
         return None, None
 
 
-def cpu_baseline(max_seconds: float = 12.0) -> dict:
-    """The oracle's mc_cpu.c restatement, all allowed host cores, bounded sample."""
+def _port_baseline(ncores: int, max_seconds: float) -> dict:
+    """The oracle's per-replica restatement of mc_cpu.c on ncores threads."""
     from oracle import oracle as O
+
+    nrep = 8 * ncores
+    g = -np.ones((nrep, L, L), dtype=np.int32)
+    t0 = time.perf_counter()
+    O.sweep_mt_per_replica(L, g, 2894203475, 20, BETA, H, nthreads=ncores)
+    dt = (time.perf_counter() - t0) / 20
+    nsweeps = int(max(1, min(200000, max_seconds / max(dt, 1e-6))))
+    g = -np.ones((nrep, L, L), dtype=np.int32)
+    t0 = time.perf_counter()
+    O.sweep_mt_per_replica(L, g, 2894203475, nsweeps, BETA, H, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    return {"value": nrep * nsweeps * L * L / dt, "unit": "attempts/s", "cores": ncores, "kind": "port",
+            "sample": f"L={L}, {nrep} replicas x {nsweeps} sweeps from all -1, beta={BETA}, h={H}, "
+                      f"per-replica MT19937 (src/mc_cpu.c arithmetic restated in oracle/), {dt:.1f} s"}
+
+
+def cpu_baseline(max_seconds: float = 12.0) -> dict:
+    """The reference's own mc_cpu.c (compiled from its sources into oracle/_ref) on all allowed host
+    cores (at most 16), one process per core (its MT19937 state is global), run in a child process
+    that does not use the GPU; the oracle's threaded restatement is reported beside it and stands
+    in when oracle/_ref was not built."""
+    import subprocess
 
     try:
         ncores = len(os.sched_getaffinity(0))
     except AttributeError:
         ncores = os.cpu_count() or 1
     ncores = max(1, min(ncores, 16))
-    nrep = 8 * ncores
-    g = -np.ones((nrep, L, L), dtype=np.int32)
-    # calibrate with one sweep, then size the sample to ~max_seconds
-    t0 = time.perf_counter()
-    O.sweep_mt_per_replica(L, g, 2894203475, 1, BETA, H, nthreads=ncores)
-    dt = time.perf_counter() - t0
-    nsweeps = int(max(1, min(200000, max_seconds / max(dt, 1e-6))))
-    g = -np.ones((nrep, L, L), dtype=np.int32)
-    t0 = time.perf_counter()
-    O.sweep_mt_per_replica(L, g, 2894203475, nsweeps, BETA, H, nthreads=ncores)
-    dt = time.perf_counter() - t0
-    attempts = nrep * nsweeps * L * L
-    return {"value": attempts / dt, "unit": "attempts/s", "cores": ncores, "kind": "port",
-            "sample": f"L={L}, {nrep} replicas x {nsweeps} sweeps from all -1, beta={BETA}, h={H}, "
-                      f"per-replica MT19937 (src/mc_cpu.c arithmetic), {dt:.1f} s"}
+    port = _port_baseline(ncores, max_seconds / 2)
+    try:
+        r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--cores", str(ncores), "--seconds",
+                            str(max_seconds), "--L", str(L)], cwd=ROOT, capture_output=True, text=True,
+                           timeout=10 * max_seconds + 60)
+        ref = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except Exception:
+        ref = None
+    if ref is None:
+        return port
+    ref["port"] = port
+    return ref
 
 
 def main() -> int:
@@ -272,7 +294,7 @@ def main() -> int:
     rng = G.RngState(R, 20251202)
     eng = G.Engine(L, R, args.method, rng, BETA, H)
     g0 = -np.ones((R, L, L), dtype=np.int32)
-    host = np.empty((R, L * L), dtype=np.int32)
+    dump = np.empty((R * L * L + 7) // 8, dtype=np.uint8)  # gridstates.bin record body
     S, split = G.get_tuning(args.method, L, R)
 
     # warmup (untimed), then restart from all -1
@@ -291,7 +313,7 @@ def main() -> int:
         c = eng.counts()  # synchronises
         frac.append(c[0] / R)
         if not args.no_dump and (step + 1) % DUMP_EVERY_STEPS == 0:
-            G.check(G.lib().gasp_engine_get_grids(eng.ptr, host.ctypes.data, 0))
+            eng.gridstates(dump)
     eng.sync()
     wall = time.perf_counter() - t0
     kern_ms = eng.kernel_ms() - ms0
@@ -302,7 +324,13 @@ def main() -> int:
     value = attempts / wall
     kern_rate = attempts / (kern_ms / 1e3)
     per_launch_ms = kern_ms / max(args.steps, 1)
-    achieved = SWEEPS_PER_STEP * L * L * R * LDS_BYTES_PER_ATTEMPT / (per_launch_ms / 1e3) / 1e9
+    # methods 1/2: LDS bytes (5 reads of the 4-byte bank word holding the spin bit); method 0: HBM
+    # bytes (5 spin bytes read per attempt)
+    hbm = args.method == 0
+    bpa = HBM_BYTES_PER_ATTEMPT if hbm else LDS_BYTES_PER_ATTEMPT
+    peak = HBM_PEAK_GBS if hbm else LDS_PEAK_GBS
+    achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
+    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: "LAT_MULTISPIN"}[args.method]
     sweeps = np.arange(1, args.steps + 1) * SWEEPS_PER_STEP
     k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
 
@@ -322,7 +350,8 @@ def main() -> int:
         "config": {
             "workload": "multispin throughput: L=64, 16384 replicas (64 per 64-bit word), "
                         "beta=0.54, h=0.07, all -1, 100 sweeps per step with fused magnetisation "
-                        "and nucleation check, int32 grid dump every 1000 sweeps",
+                        "and nucleation check, bit-packed gridstates.bin record (8.4 MB) to host memory "
+                        "every 1000 sweeps",
             "L": L, "replicas": R, "sweeps_per_step": SWEEPS_PER_STEP,
             "total_sweeps": args.steps * SWEEPS_PER_STEP, "gpu_method": args.method,
             "rng": "philox4x32-10 per replica", "slots_per_replica": S,
@@ -330,14 +359,14 @@ def main() -> int:
             "grid_dumps": not args.no_dump,
         },
         "roofline": {
-            "bound": "lds",
+            "bound": "hbm" if hbm else "lds",
             "achieved": achieved,
-            "peak": LDS_PEAK_GBS,
+            "peak": peak,
             "unit": "GB/s",
-            "frac": achieved / LDS_PEAK_GBS,
+            "frac": achieved / peak,
             "traffic": None,
-            "bytes_per_attempt": LDS_BYTES_PER_ATTEMPT,
-            "kernel": "sweep_kernel<LAT_MULTISPIN, RNG_PHILOX, S>",
+            "bytes_per_attempt": bpa,
+            "kernel": f"sweep_kernel<{kname}, RNG_PHILOX, S>",
             "kernel_ms_per_launch": per_launch_ms,
             "kernel_attempts_per_s": kern_rate,
         },
@@ -345,7 +374,7 @@ def main() -> int:
                            "reference_k_per_sweep": 1.5032e-4},
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic_sweep_multispin_L64.json")
-    if os.path.exists(traffic_file):
+    if args.method == 2 and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             prof = json.load(f)
         out["roofline"]["traffic"] = prof.get("hbm_bytes_per_launch")

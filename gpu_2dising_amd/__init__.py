@@ -29,7 +29,7 @@ EXPORTS = (
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
-    "gasp_engine_gather_grids",
+    "gasp_engine_gather_grids", "gasp_engine_get_gridstates",
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
@@ -139,6 +139,8 @@ def lib() -> C.CDLL:
     L.gasp_engine_get_grids.argtypes = [vp, vp, C.c_int]
     L.gasp_engine_gather_grids.argtypes = [vp, vp, C.c_int, vp]
     L.gasp_engine_gather_grids.restype = C.c_int
+    L.gasp_engine_get_gridstates.argtypes = [vp, vp]
+    L.gasp_engine_get_gridstates.restype = C.c_int
     L.gasp_engine_set_inputs.argtypes = [vp, vp, C.c_int]
     L.gasp_engine_set_freeze.argtypes = [vp, C.c_int]
     L.gasp_engine_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
@@ -246,6 +248,15 @@ class Engine:
     def get_grids(self) -> np.ndarray:
         out = np.empty((self.R, self.L, self.L), dtype=np.int32)
         check(lib().gasp_engine_get_grids(self.ptr, out.ctypes.data, 0))
+        return out
+
+    def gridstates(self, out: np.ndarray = None) -> np.ndarray:
+        """All grids as a gridstates.bin record body (uint8, ceil(R*L*L/8) bytes)."""
+        n = (self.R * self.L * self.L + 7) // 8
+        if out is None:
+            out = np.empty(n, dtype=np.uint8)
+        assert out.dtype == np.uint8 and out.size == n and out.flags.c_contiguous
+        check(lib().gasp_engine_get_gridstates(self.ptr, out.ctypes.data))
         return out
 
     def gather(self, replicas) -> np.ndarray:
@@ -463,7 +474,10 @@ def run_driver(L: int, grids: np.ndarray, *, beta: float, h: float, rng: RngStat
     f = McFunction(itask, dn_thr, up_thr, ninputs, initial_spin, cv.encode(), _fp(result),
                    b"gridstates.bin")
     run = GpuRun(rng.ptr, threads_per_block, method)
-    cb = GridOutputFunc(outfunc) if outfunc is not None else GridOutputFunc(0)
+    if outfunc == "write_ising_grids":  # the library's own gridstates.bin writer
+        cb = GridOutputFunc(C.cast(Lb.write_ising_grids, C.c_void_p).value)
+    else:
+        cb = GridOutputFunc(outfunc) if outfunc is not None else GridOutputFunc(0)
     try:
         Lb.mc_driver_gpu(g, beta, h, _ip(fates), s, f, run, cb)
         check()

@@ -42,6 +42,7 @@ constexpr int OFF_W[5] = {0, 0, 320, 9568, 18816};     // conv k = 2..4
 constexpr int OFF_B[5] = {0, 288, 9536, 18784, 28032};
 constexpr int OFF_FW1 = 28064, OFF_FB1 = 30112, OFF_FW2 = 30176, OFF_FB2 = 30240;
 constexpr int HEAD_N = NPARAM - OFF_FW1;               // 2177
+constexpr int kRedChunk = 32;                          // partials per first-pass chunk
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -115,17 +116,26 @@ __global__ void conv1_fwd(const int8_t *__restrict__ s, const float *__restrict_
 }
 
 // ------------------------------------------------------------------ conv 32->32
-// One wave per image row: the row's L/16 tiles of 16 positions x 32 output
-// channels.  BWD = false: out = bf16(relu(acc + bias)); BWD = true (backward
-// data with the flipped/transposed weights): out = bf16(acc * (mask > 0)).
+// A workgroup of 4 waves computes 4 consecutive rows of one image (L is a
+// multiple of 4); wave w owns row y0 + w, in 16-position tiles x 32 output
+// channels.  The input band (rows y0 - 1 .. y0 + 4, periodic) is staged in LDS
+// one x-chunk of up to 64 positions (+ 1 halo position each side) at a time,
+// so each input element is read from memory about 1.5 times instead of once
+// per tap; the 9 taps then take their B fragments from LDS.  LDS positions
+// are 80 bytes apart (32 channels + 8 pad), which makes the 16 lanes of a
+// 16-byte fragment read start on 16 different bank quads.
+// BWD = false: out = bf16(relu(acc + bias)); BWD = true (backward data with
+// the flipped/transposed weights): out = bf16(acc * (mask > 0)).
+constexpr int kConvRows = 4, kConvXT = 64, kConvPS = 40;  // rows per block, x-chunk, position stride (bf16)
+
 template <bool BWD>
 __global__ void __launch_bounds__(256) conv_mfma(const uint16_t *__restrict__ in, const uint16_t *__restrict__ wfrag,
                                                  const float *__restrict__ bias, const uint16_t *__restrict__ mask,
                                                  uint16_t *__restrict__ out, int n, int L) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);  // global row over all images
-    if (row >= n * L) return;                             // wave-uniform
-    const int img = row / L, y = row % L, m = L - 1, N = L * L;
+    __shared__ __align__(16) uint16_t tile[(kConvRows + 2) * (kConvXT + 2) * kConvPS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * kConvRows;  // grid = n * L / kConvRows exactly
+    const int img = row0 / L, y0 = row0 % L, m = L - 1, N = L * L;
     uint4 wa[9][2];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -133,33 +143,47 @@ __global__ void __launch_bounds__(256) conv_mfma(const uint16_t *__restrict__ in
         for (int h = 0; h < 2; ++h) wa[t][h] = ((const uint4 *)wfrag)[(t * 2 + h) * 64 + lane];
     const int pl = lane & 15, kq = lane >> 4;
     const uint16_t *base = in + (size_t)img * N * C;
-    for (int x0 = 0; x0 < L; x0 += 16) {
-        f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int yy = (y + t / 3 - 1) & m, xx = (x0 + pl + t % 3 - 1) & m;
-            const uint4 b = *(const uint4 *)(base + ((size_t)(yy * L + xx) * C + 8 * kq));
-            acc[0] = mfma(wa[t][0], b, acc[0]);
-            acc[1] = mfma(wa[t][1], b, acc[1]);
+    const int xw = L < kConvXT ? L : kConvXT;
+    const int y = y0 + w;
+    for (int xc = 0; xc < L; xc += xw) {
+        __syncthreads();  // the previous chunk's fragments have been read
+        const int items = (kConvRows + 2) * (xw + 2) * 4;  // 16-byte pieces: 4 per position
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
+            const int q = it & 3, p = (it >> 2) % (xw + 2), r = (it >> 2) / (xw + 2);
+            const int yy = (y0 + r - 1) & m, xx = (xc + p - 1) & m;
+            *(uint4 *)&tile[(r * (kConvXT + 2) + p) * kConvPS + 8 * q] =
+                *(const uint4 *)(base + ((size_t)(yy * L + xx) * C + 8 * q));
         }
-        const size_t opos = (size_t)img * N + (size_t)y * L + x0 + pl;
+        __syncthreads();
+        for (int x0 = 0; x0 < xw; x0 += 16) {
+            f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c0 = 16 * h + 4 * kq;
-            float v[4];
-            if constexpr (BWD) {
-                const uint2 mk = *(const uint2 *)(mask + opos * C + c0);
-                const uint16_t mb[4] = {(uint16_t)mk.x, (uint16_t)(mk.x >> 16), (uint16_t)mk.y, (uint16_t)(mk.y >> 16)};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = bf2f(mb[j]) > 0.0f ? acc[h][j] : 0.0f;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = fmaxf(acc[h][j] + bias[c0 + j], 0.0f);
+            for (int t = 0; t < 9; ++t) {
+                const int r = w + t / 3, p = x0 + pl + t % 3;
+                const uint4 b = *(const uint4 *)&tile[(r * (kConvXT + 2) + p) * kConvPS + 8 * kq];
+                acc[0] = mfma(wa[t][0], b, acc[0]);
+                acc[1] = mfma(wa[t][1], b, acc[1]);
             }
-            uint2 o;
-            o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-            o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-            *(uint2 *)(out + opos * C + c0) = o;
+            const size_t opos = (size_t)img * N + (size_t)y * L + xc + x0 + pl;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c0 = 16 * h + 4 * kq;
+                float v[4];
+                if constexpr (BWD) {
+                    const uint2 mk = *(const uint2 *)(mask + opos * C + c0);
+                    const uint16_t mb[4] = {(uint16_t)mk.x, (uint16_t)(mk.x >> 16), (uint16_t)mk.y,
+                                            (uint16_t)(mk.y >> 16)};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = bf2f(mb[j]) > 0.0f ? acc[h][j] : 0.0f;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] = fmaxf(acc[h][j] + bias[c0 + j], 0.0f);
+                }
+                uint2 o;
+                o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *(uint2 *)(out + opos * C + c0) = o;
+            }
         }
     }
 }
@@ -380,6 +404,17 @@ __global__ void reduce_partials(const float *__restrict__ part, int P, int M, fl
     out[j] = s;
 }
 
+// First pass of a two-level deterministic reduction: tmp[c][j] = sum of
+// part[p][j] over the chunk p in [c * chunk, (c + 1) * chunk).
+__global__ void reduce_chunks(const float *__restrict__ part, int P, int M, int chunk, float *__restrict__ tmp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const int p0 = blockIdx.y * chunk, p1 = min(P, p0 + chunk);
+    float s = 0.f;
+    for (int i = p0; i < p1; ++i) s += part[(size_t)i * M + j];
+    tmp[(size_t)blockIdx.y * M + j] = s;
+}
+
 // Adam with bias correction, float32 master weights
 __global__ void adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                      int n, float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -421,6 +456,7 @@ struct gasp_cnn {
     uint16_t *a[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // a[1..4]
     uint16_t *dz0 = nullptr, *dz1 = nullptr;
     float *part = nullptr, *part_b = nullptr, *part1 = nullptr, *part_head = nullptr;
+    float *red = nullptr;  // chunk sums of the two-level reductions
     int8_t *data = nullptr;
     float *data_y = nullptr;
     int *order = nullptr;
@@ -449,7 +485,7 @@ extern "C" int gasp_cnn_destroy(gasp_cnn *c) {
                     (void *)c->wb, (void *)c->s, (void *)c->y, (void *)c->pb, (void *)c->loss, (void *)c->dg,
                     (void *)c->a[1], (void *)c->a[2], (void *)c->a[3], (void *)c->a[4], (void *)c->dz0,
                     (void *)c->dz1, (void *)c->part, (void *)c->part_b, (void *)c->part1, (void *)c->part_head,
-                    (void *)c->data, (void *)c->data_y, (void *)c->order})
+                    (void *)c->red, (void *)c->data, (void *)c->data_y, (void *)c->order})
         cfree(q);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -491,6 +527,11 @@ extern "C" int gasp_cnn_create(int L, int max_batch, gasp_cnn **out) {
     A((void **)&c->part_b, n * 4 * C * 4);
     A((void **)&c->part1, n * (size_t)L * 320 * 4);
     A((void **)&c->part_head, n * HEAD_N * 4);
+    {
+        const size_t q = (size_t)kRedChunk;
+        const size_t r = std::max({(n * 4 + q - 1) / q * 9 * C * C, (n * L + q - 1) / q * 320, (n + q - 1) / q * HEAD_N});
+        A((void **)&c->red, r * 4);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) {
         gasp_cnn_destroy(c);
@@ -498,6 +539,23 @@ extern "C" int gasp_cnn_create(int L, int max_batch, gasp_cnn **out) {
                          hipGetErrorString(e));
     }
     *out = c;
+    return GASP_OK;
+}
+
+// out[j] = sum_p part[p][j] for part [P][M], in a fixed order: chunks of
+// kRedChunk partials in parallel, then the chunk sums.
+static int reduce_to(gasp_cnn *c, const float *part, int P, int M, float *out) {
+    const unsigned gx = (unsigned)((M + 255) / 256);
+    if (P <= kRedChunk) {
+        hipLaunchKernelGGL(reduce_partials, dim3(gx), dim3(256), 0, c->st, part, P, M, out);
+    } else {
+        // at least kRedChunk per chunk (the buffer holds ceil(P / kRedChunk) x M), and at most 65535 chunks
+        const int chunk = std::max(kRedChunk, (P + 65534) / 65535);
+        const int nc = (P + chunk - 1) / chunk;
+        hipLaunchKernelGGL(reduce_chunks, dim3(gx, (unsigned)nc), dim3(256), 0, c->st, part, P, M, chunk, c->red);
+        hipLaunchKernelGGL(reduce_partials, dim3(gx), dim3(256), 0, c->st, (const float *)c->red, nc, M, out);
+    }
+    CTRY(hipGetLastError());
     return GASP_OK;
 }
 
@@ -537,7 +595,7 @@ static int forward(gasp_cnn *c, int n, const float *d_labels) {
                        c->a[1], n, L);
     const int rows = n * L;
     for (int k = 2; k <= 4; ++k)
-        hipLaunchKernelGGL(conv_mfma<false>, dim3((rows + 3) / 4), dim3(256), 0, c->st, c->a[k - 1],
+        hipLaunchKernelGGL(conv_mfma<false>, dim3(rows / kConvRows), dim3(256), 0, c->st, c->a[k - 1],
                            c->wf + (size_t)(k - 2) * 9 * 2 * 64 * 8, c->p + OFF_B[k], (const uint16_t *)nullptr, c->a[k],
                            n, L);
     hipLaunchKernelGGL(head, dim3(n), dim3(256), 0, c->st, c->a[4], c->p, d_labels, c->pb, c->loss, c->part_head,
@@ -551,23 +609,22 @@ static int backward(gasp_cnn *c, int n) {
     const int L = c->L, N = c->N;
     const size_t tot = (size_t)n * N;
     const int rows = n * L;
-    hipLaunchKernelGGL(reduce_partials, dim3((HEAD_N + 255) / 256), dim3(256), 0, c->st, c->part_head, n, HEAD_N,
-                       c->g + OFF_FW1);
+    int rc = reduce_to(c, c->part_head, n, HEAD_N, c->g + OFF_FW1);
+    if (rc) return rc;
     hipLaunchKernelGGL(head_bwd_fill, dim3((unsigned)((tot * 16 + 255) / 256)), dim3(256), 0, c->st, c->a[4], c->dg,
                        c->dz0, n, L);
     uint16_t *dz = c->dz0, *dzn = c->dz1;
     for (int k = 4; k >= 2; --k) {
         hipLaunchKernelGGL(conv_wgrad, dim3(n, 9), dim3(256), 0, c->st, c->a[k - 1], dz, c->part, c->part_b, n, L);
-        hipLaunchKernelGGL(reduce_partials, dim3((9 * C * C + 255) / 256), dim3(256), 0, c->st, c->part, n * 4,
-                           9 * C * C, c->g + OFF_W[k]);
-        hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, c->st, c->part_b, n * 4, C, c->g + OFF_B[k]);
+        if ((rc = reduce_to(c, c->part, n * 4, 9 * C * C, c->g + OFF_W[k])) != 0) return rc;
+        if ((rc = reduce_to(c, c->part_b, n * 4, C, c->g + OFF_B[k])) != 0) return rc;
         // d a_{k-1}, masked by a_{k-1} > 0
-        hipLaunchKernelGGL(conv_mfma<true>, dim3((rows + 3) / 4), dim3(256), 0, c->st, dz,
+        hipLaunchKernelGGL(conv_mfma<true>, dim3(rows / kConvRows), dim3(256), 0, c->st, dz,
                            c->wb + (size_t)(k - 2) * 9 * 2 * 64 * 8, (const float *)nullptr, c->a[k - 1], dzn, n, L);
         std::swap(dz, dzn);
     }
     hipLaunchKernelGGL(conv1_wgrad, dim3(rows), dim3(256), 0, c->st, c->s, dz, c->part1, n, L);
-    hipLaunchKernelGGL(reduce_partials, dim3((320 + 255) / 256), dim3(256), 0, c->st, c->part1, rows, 320, c->g);
+    if ((rc = reduce_to(c, c->part1, rows, 320, c->g)) != 0) return rc;
     CTRY(hipGetLastError());
     return GASP_OK;
 }

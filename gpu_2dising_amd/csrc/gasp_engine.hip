@@ -810,6 +810,26 @@ extern "C" int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device) 
     return rc ? rc : engine_sync(e);
 }
 
+extern "C" int gasp_engine_get_gridstates(gasp_engine *e, uint8_t *host_out) {
+    clear_error();
+    if (!e || !host_out) return set_error(GASP_ERR_ARG, "gasp_engine_get_gridstates: NULL argument");
+    const size_t nbytes = ((size_t)e->R * e->N + 7) / 8;
+    uint8_t *d = nullptr;
+    hipError_t he = hipMalloc(&d, nbytes);
+    if (he == hipSuccess) {
+        hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+                           e->stride, e->R, e->N, d, nbytes);
+        he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(host_out, d, nbytes, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    dfree(d);
+    if (he != hipSuccess)
+        return set_error(he == hipErrorOutOfMemory ? GASP_ERR_NOMEM : GASP_ERR_HIP, "GPUassert: %s %s %d",
+                         hipGetErrorString(he), __FILE__, __LINE__);
+    return engine_collect_events(e);
+}
+
 extern "C" int gasp_engine_gather_grids(gasp_engine *e, const int *replicas, int n, int8_t *host_out) {
     clear_error();
     if (!e || (n > 0 && (!replicas || !host_out)) || n < 0)
@@ -1333,6 +1353,18 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
                  : set_error(GASP_ERR_HIP, "GPUassert: fate upload");
     // collective variable of the initial state (isweep = 0 is always a check)
     if (!rc) rc = launch_sweep(e, 0, nullptr, use_cluster ? CHECK_NONE : check_mode, dn, up);
+    // With the library's own gridstates.bin writer as the output function, grid
+    // dumps are packed to bits on the device and only the bits are copied and
+    // appended (the same records); the caller's host array receives the last
+    // dumped state at the end, which is what the reference leaves in it.
+    const bool packed_out = outfunc == write_ising_grids;
+    const size_t pbytes = ((size_t)R * L * L + 7) / 8;
+    uint8_t *d_bits = nullptr, *h_bits = nullptr;
+    bool dumped = false;
+    if (!rc && packed_out) {
+        if (hipMalloc(&d_bits, pbytes) != hipSuccess || hipHostMalloc(&h_bits, pbytes) != hipSuccess)
+            rc = set_error(GASP_ERR_NOMEM, "Error: no memory for grid dumps");
+    }
     const double t_ms0 = e->kernel_ms;
     const auto t0 = std::chrono::steady_clock::now();
     const bool py = output_style() == 1;
@@ -1346,7 +1378,15 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
             if ((rc = engine_cluster(e, -calc.initial_spin))) break;
             if ((rc = engine_check_cv(e, e->d_lclus, check_mode, dn, up))) break;
         }
-        if (do_grid) {
+        if (do_grid && packed_out) {
+            hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+                               e->stride, R, e->N, d_bits, pbytes);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(h_bits, d_bits, pbytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess) {
+                rc = set_error(GASP_ERR_HIP, "GPUassert: grid download");
+                break;
+            }
+        } else if (do_grid) {
             if ((rc = engine_unpack<int>(e, grids.d_ising_grids))) break;
             if (hipMemcpyAsync(grids.ising_grids, grids.d_ising_grids, (size_t)R * L * L * sizeof(int),
                                hipMemcpyDeviceToHost, e->stream) != hipSuccess) {
@@ -1375,9 +1415,14 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
             if ((rc = engine_advance(e, (uint64_t)spc * (uint64_t)e->N, chk, dn, up))) break;
         }
         if (do_grid || isweep == tot - 1) {
-            if (outfunc)
+            if (packed_out) {
+                // return value ignored, as the reference ignores outfunc's
+                append_gridstates(L, R, isweep, h_bits, calc.filename);
+                dumped = true;
+            } else if (outfunc) {
                 outfunc(L, R, grids.ising_grids, isweep, magnetisation.data(), use_cluster ? lclus.data() : nullptr,
                         calc.cv, dn, up, calc.filename);
+            }
         }
         if (report) {
             if (itask == 0) {
@@ -1414,6 +1459,12 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
     t_stats.waves_per_group = e->W;
     t_stats.method = e->method;
     engine_free(e);
+    if (!rc && dumped) {
+        const size_t n = (size_t)R * L * L;
+        for (size_t i = 0; i < n; ++i) grids.ising_grids[i] = ((h_bits[i >> 3] >> (i & 7)) & 1) ? 1 : -1;
+    }
+    dfree(d_bits);
+    if (h_bits) (void)hipHostFree(h_bits);
     if (rc) return;
 
     if (itask == 0) {

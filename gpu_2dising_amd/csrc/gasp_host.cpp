@@ -120,21 +120,14 @@ int write_ising_grids(int L, int ngrids, int *ising_grids, int isweep, float *ma
     (void)dn_thr;
     (void)up_thr;
     gasp::clear_error();
-    const char *name = filename ? filename : "gridstates.bin";
-    FILE *f = fopen(name, "ab");
-    if (!f) f = fopen(name, "wb");
-    if (!f) return set_error(GASP_ERR_IO, "Error opening %s for write!", name);
-    int hdr[3] = {L, ngrids, isweep};
     const size_t n = (size_t)L * L * ngrids;
     const size_t nbytes = (n + 7) / 8;
     std::vector<unsigned char> bits(nbytes, 0);
     for (size_t i = 0; i < n; ++i)
         if (ising_grids[i] == 1) bits[i >> 3] |= (unsigned char)(1u << (i & 7));
-    const bool ok = fwrite(hdr, sizeof(int), 3, f) == 3 && fwrite(bits.data(), 1, nbytes, f) == nbytes;
-    fclose(f);
-    if (!ok) return set_error(GASP_ERR_IO, "Error writing %s", name);
-    return 0;
+    return gasp::append_gridstates(L, ngrids, isweep, bits.data(), filename);
 }
+
 
 // src/bootstrap.c:6-63.  m resampled committors over ngrids fates, indices
 // drawn with libc rand() % ngrids before counting; the population standard
@@ -168,3 +161,17 @@ double bootstrap_errbar(int ngrids, int m, int *grid_fate_host) {
 }
 
 }  // extern "C"
+
+int gasp::append_gridstates(int L, int ngrids, int isweep, const uint8_t *bits, const char *filename) {
+    const char *name = filename ? filename : "gridstates.bin";
+    FILE *f = fopen(name, "ab");
+    if (!f) f = fopen(name, "wb");
+    if (!f) return set_error(GASP_ERR_IO, "Error opening %s for write!", name);
+    const int hdr[3] = {L, ngrids, isweep};
+    const size_t nbytes = ((size_t)L * L * ngrids + 7) / 8;
+    const bool ok = fwrite(hdr, sizeof(int), 3, f) == 3 && fwrite(bits, 1, nbytes, f) == nbytes;
+    fclose(f);
+    if (!ok) return set_error(GASP_ERR_IO, "Error writing %s", name);
+    return 0;
+}
+

@@ -23,6 +23,10 @@ int output_style();
 // Exact integer threshold T of "(u + 0.5) 2^-32 < p" (u < T), T in [0, 2^32].
 uint64_t threshold_of(double p);
 
+// Appends one gridstates.bin record (int32 L, ngrids, isweep + the packed
+// bits, src/io.c:67-130) to `filename` (NULL: gridstates.bin).
+int append_gridstates(int L, int ngrids, int isweep, const uint8_t *bits, const char *filename);
+
 }  // namespace gasp
 
 extern "C" {

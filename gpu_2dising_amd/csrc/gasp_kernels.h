@@ -96,6 +96,12 @@ struct Stencil {
 __device__ __forceinline__ bool in_stencil(int x, const Stencil &s) {
     return (x == s.c) | (x == s.up) | (x == s.dn) | (x == s.lt) | (x == s.rt);
 }
+// Wave mask of the lanes whose stencil holds the (wave-uniform) site x: one
+// compare-to-mask per stencil site, ORed as scalars.
+__device__ __forceinline__ uint64_t ballot_in_stencil(int x, const Stencil &s) {
+    return __ballot(x == s.c) | __ballot(x == s.up) | __ballot(x == s.dn) | __ballot(x == s.lt) |
+           __ballot(x == s.rt);
+}
 
 template <bool POW2>
 struct Geometry {
@@ -487,13 +493,13 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                 const uint64_t later = grp & ~((2ull << i) - 1ull);  // i = 63: 2 << 63 = 0, later = 0
                 if ((mA >> i) & 1ull) {
                     const int x = __builtin_amdgcn_readlane(A.st.c, i);
-                    cA |= __ballot(in_stencil(x, A.st)) & later;
-                    cB |= __ballot(in_stencil(x, B.st)) & later;
+                    cA |= ballot_in_stencil(x, A.st) & later;
+                    cB |= ballot_in_stencil(x, B.st) & later;
                 }
                 if ((mB >> i) & 1ull) {
                     const int x = __builtin_amdgcn_readlane(B.st.c, i);
-                    cA |= __ballot(in_stencil(x, A.st)) & later;
-                    cB |= __ballot(in_stencil(x, B.st)) & later;
+                    cA |= ballot_in_stencil(x, A.st) & later;
+                    cB |= ballot_in_stencil(x, B.st) & later;
                 }
             }
             const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << gbase) - 1ull);
@@ -761,6 +767,33 @@ __global__ void __launch_bounds__(64) mt_draws_kernel(uint32_t *st, int R, uint6
 // ------------------------------------------------------------ small kernels
 // (kernels rather than memset / host updates, so that they can be captured in
 // a graph and replayed)
+// The body of a gridstates.bin record (src/io.c:67-130): the R*N spins as one
+// bit stream, replica-major then site-major, bit k of byte b = spin 8b + k
+// (1 = up), padded with zero bits to whole bytes.  One thread per output byte.
+__global__ void pack_gridstates(const void *__restrict__ lat, int method, int stride, int R, int N,
+                                uint8_t *__restrict__ out, size_t nbytes) {
+    const size_t total = (size_t)R * N;
+    for (size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x; b < nbytes; b += (size_t)gridDim.x * blockDim.x) {
+        unsigned v = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const size_t i = 8 * b + k;
+            if (i >= total) break;
+            const int r = (int)(i / N), site = (int)(i - (size_t)r * N);
+            unsigned up;
+            if (method == 2) {
+                up = (unsigned)((((const uint64_t *)lat)[(size_t)(r >> 6) * stride + site] >> (r & 63)) & 1ull);
+            } else if (method == 1) {
+                up = (((const unsigned *)lat)[(size_t)r * stride + (site >> 5)] >> (site & 31)) & 1u;
+            } else {
+                up = ((const unsigned char *)lat)[(size_t)r * stride + site];
+            }
+            v |= up << k;
+        }
+        out[b] = (uint8_t)v;
+    }
+}
+
 // Selected replicas idx[0..n) as int8 +-1 grids [n][N], from any native layout.
 __global__ void gather_spins8(const void *__restrict__ lat, int method, int stride, const int *__restrict__ idx,
                               int n, int N, int8_t *__restrict__ out) {

@@ -216,6 +216,34 @@ def estimate(grids, params, *, lo=0.2, hi=0.8, reps=128, seed=303, max_sweeps=20
                          "seconds": time.perf_counter() - t0}
 
 
+FWD_FLOP = 2.29e8  # per L=64 grid: conv1 2.36 MFLOP + 3 x 75.5 MFLOP (32 -> 32, 3x3)
+
+
+def bench(params, n: int = 4096, train_batch: int = 256, seed: int = 5) -> dict:
+    """Inference (one batch of n) and training-epoch throughput of the CNN at L = 64."""
+    grids = disc_grids(n, 64, seed=seed).astype(np.int8)
+    out = {}
+    with Surrogate(64, n) as net:
+        net.set_params(params)
+        net.predict(grids)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            net.predict(grids)
+        dt = (time.perf_counter() - t0) / 5
+        out.update(predict_grids_per_s=n / dt, predict_ms=dt * 1e3, forward_tflops=n * FWD_FLOP / dt / 1e12)
+    with Surrogate(64, train_batch) as net:
+        net.set_params(params)
+        net.set_data(grids, np.random.default_rng(seed).random(n).astype(np.float32))
+        order = np.arange(n, dtype=np.int32)
+        net.train_epoch(order, train_batch, 1e-4)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            net.train_epoch(order, train_batch, 1e-4)
+        dt = (time.perf_counter() - t0) / 3
+        out.update(train_grids_per_s=n / dt, train_tflops=n * 3 * FWD_FLOP / dt / 1e12)
+    return out
+
+
 def main():
     import argparse
 
@@ -226,7 +254,14 @@ def main():
     ap.add_argument("--reps", type=int, default=256)
     ap.add_argument("--sweeps", type=int, default=50000)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--bench", action="store_true",
+                    help="only time inference and training with the packaged parameters")
     args = ap.parse_args()
+    if args.bench:
+        init_device(-1)
+        here = os.path.dirname(os.path.abspath(__file__))
+        print(json.dumps(bench(np.load(os.path.join(here, "surrogate_L64.npz"))["params"])), flush=True)
+        return
     os.makedirs(args.out, exist_ok=True)
     init_device(-1)
     set_tuning(0, 0)

@@ -216,6 +216,11 @@ int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device);
  * host memory: the int8 snapshot copies of the Python module's grid history
  * (src/ising_ext.cu:280-463) without moving every replica's int32 grid. */
 int gasp_engine_gather_grids(gasp_engine *e, const int *replicas, int n, int8_t *host_out);
+/* All grids as the body of a gridstates.bin record (write_ising_grids,
+ * src/io.c:67-130): ceil(R*L*L / 8) bytes, replica-major, site-major, bit k of
+ * byte b = spin 8b + k (1 = up), packed on the device so that only the bits
+ * cross to the host. */
+int gasp_engine_get_gridstates(gasp_engine *e, uint8_t *host_out);
 /* Advance every replica by nsweeps sweeps.  check: 0 none, 1 count replicas
  * with m > up_thr, 2 update fates (sticky; 1 if m > up_thr, 0 if
  * m < dn_thr).  The magnetisation after the sweeps is always computed. */

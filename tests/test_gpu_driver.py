@@ -154,3 +154,33 @@ def test_driver_argument_errors(gpu, engine_lib):
                              grid_output_int=5, **kw)
     finally:
         rng.free()
+
+
+def test_gridstates_dump_fast_path(gpu, engine_lib, tmp_path, monkeypatch):
+    """With write_ising_grids itself as the output function the driver packs
+    dumps on the device; the file and the caller's host grids must equal those
+    of the generic path (a callback that calls write_ising_grids)."""
+    import ctypes as C
+
+    G = engine_lib
+    monkeypatch.chdir(tmp_path)
+    L, R, seed = 16, 128, 4242
+    lib = G.lib()
+    print("[driver fast dump]", flush=True)
+
+    def generic(Lc, n, grid, isweep, mag, lclus, cv, dn, up, fn):
+        return lib.write_ising_grids(Lc, n, grid, isweep, mag, lclus, cv, dn, up, fn)
+
+    outs = []
+    for outfunc in (generic, "write_ising_grids"):
+        rng = G.RngState(R, seed)
+        res = G.run_driver(L, -np.ones((R, L, L), dtype=np.int32), beta=0.54, h=0.07, rng=rng, method=2,
+                           tot_nsweeps=310, mag_output_int=20, grid_output_int=40, outfunc=outfunc)
+        rng.free()
+        outs.append((open("gridstates.bin", "rb").read(), res["host_grids"].copy(), res["result"]))
+        os.remove("gridstates.bin")
+    (fa, ha, ra), (fb, hb, rb) = outs
+    rec = 12 + R * L * L // 8
+    assert len(fa) > 0 and len(fa) % rec == 0  # dumps every 40 sweeps until the run ends
+    assert fa == fb
+    assert np.array_equal(ha, hb) and np.array_equal(ra, rb)

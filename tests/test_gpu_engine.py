@@ -178,3 +178,29 @@ def test_gather_grids_int8(gpu, engine_lib, method):
     rng.free()
     assert got.dtype == np.int8
     assert np.array_equal(got, full[idx].astype(np.int8))
+
+
+@pytest.mark.parametrize("method,L", [(0, 17), (1, 16), (2, 17), (2, 64)])
+def test_gridstates_record_body(gpu, engine_lib, tmp_path, method, L):
+    """gasp_engine_get_gridstates equals write_ising_grids' bit packing of the
+    engine's int32 grids (src/io.c:67-130), also when L*L is not a multiple of 8."""
+    import ctypes as C
+
+    G = engine_lib
+    R = 70
+    print(f"[gridstates] method={method} L={L}", flush=True)
+    rng = G.RngState(R, 3)
+    with G.Engine(L, R, method, rng, 0.3, 0.0) as e:
+        e.set_grids(_start("random", R, L, 9))
+        e.sweep(2)
+        full = e.get_grids()
+        body = e.gridstates()
+    rng.free()
+    exp = np.packbits((full.reshape(-1) > 0).astype(np.uint8), bitorder="little")
+    assert np.array_equal(body, exp)
+    # the same bytes as the record write_ising_grids appends (after its 12-byte header)
+    lib = G.lib()
+    fn = str(tmp_path / "g.bin").encode()
+    assert lib.write_ising_grids(L, R, full.ctypes.data_as(C.POINTER(C.c_int)), 0, None, None,
+                                 b"magnetisation", -0.9, 0.9, fn) == 0
+    assert open(fn, "rb").read()[12:] == body.tobytes()
