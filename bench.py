@@ -397,6 +397,15 @@ def main() -> int:
         cb = cpu_baseline()
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+        if "committor" in out:
+            # The reference's CPU committor (src/mc_cpu.c:190-267) sweeps every replica of every
+            # input until all fates are resolved, so one pB estimate costs sweeps x replicas x N
+            # attempts at the CPU baseline's attempt rate (same mc_sweep_cpu arithmetic).
+            cm = out["committor"]
+            ape = float(cm["sweeps"]) * cm["replicas_per_input"] * L * L
+            cm["cpu_value"] = cb["value"] / ape
+            cm["cpu_basis"] = (f"cpu_baseline attempts/s / {ape:.4g} attempts per estimate "
+                               f"({cm['sweeps']} sweeps x {cm['replicas_per_input']} replicas x {L * L} sites)")
     print(json.dumps(out), flush=True)
     return 0
 
