@@ -221,7 +221,7 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
 
 static int pow2_ceil(long long x) {
     int p = 1;
-    while (p < x && p < 64) p <<= 1;
+    while (p < x && p < 256) p <<= 1;
     return p;
 }
 
@@ -240,6 +240,7 @@ static size_t lds_bytes_for(int method, int N, int S, int waves) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
     }
+    if (S > 64) return (size_t)kLdsWaveList + (size_t)waves * (128 * 4 + 8);  // sweep_hbm_waves lists
     return (size_t)kLdsLattice;
 }
 
@@ -266,7 +267,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     int W;
     if (method == 2) W = (g_tune_waves == 2 || g_tune_slots > 16) ? 2 : 1;
     else W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
-    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : 64;
+    // method 0 may spread a replica over 2 or 4 waves (sweep_hbm_waves)
+    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : (method == 0 ? 256 : 64);
     int S;
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), Smax);
@@ -278,9 +280,12 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
             // per replica keeps fewer replicas resident at a time, so their
             // lattices fit the L2s.  At L=64 with 16384 replicas this takes
             // a launch from 92 ms (16 lanes) to 39 ms.
-            S = 64;
+            S = std::max(S, 64);
         }
+        // several waves per replica only where conflicts are rare
+        if (method == 0 && N < 16384) S = std::min(S, 64);
     }
+    if (method == 0 && S > 64) W = S / 64;  // one replica per workgroup
     if (method == 1) {
         // enough slots so that the replica regions of one workgroup fit
         while (S < 64 && lds_bytes_for(1, N, S, W) > lds_block) S <<= 1;
@@ -320,6 +325,10 @@ static SweepFn pick_sweep(int S) {
     }
     if constexpr (LAT != LAT_MULTISPIN && LAT != LAT_MULTIHALF) {
         if (S == 64) return sweep_kernel<LAT, RNG, 64, P2>;
+    }
+    if constexpr (LAT == LAT_HBM) {
+        if (S == 128) return sweep_hbm_waves<RNG, 2, P2>;
+        if (S == 256) return sweep_hbm_waves<RNG, 4, P2>;
     }
     return nullptr;
 }
@@ -1141,7 +1150,8 @@ extern "C" double gasp_graph_ms(gasp_graph *g) { return g ? g->ms : 0.0; }
 // Times each candidate launch shape on a throw-away engine (all spins -1,
 // Philox, probe_sweeps sweeps after one warm-up sweep) and keeps the fastest
 // for (method, L, ngrids).  Candidates: slots per replica S (lanes that
-// speculate on one replica) and, for methods 0/1, waves per workgroup W.
+// speculate on one replica) and, for methods 0/1, waves per workgroup W; for
+// method 0 also 128 and 256 slots (a replica over 2 or 4 waves).
 extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
                              double *best_rate) {
     clear_error();
@@ -1156,6 +1166,10 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
         } else {
             for (int W = 1; W <= 4; W <<= 1) cand.push_back({S, W});
         }
+    }
+    if (method == 0) {
+        cand.push_back({128, 2});
+        cand.push_back({256, 4});
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();

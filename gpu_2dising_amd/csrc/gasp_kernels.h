@@ -606,6 +606,187 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     }
 }
 
+// ------------------------------------------------------------ multi-wave HBM sweep
+// Layout 0 with S = 64 W lanes per replica spread over the W waves (2 or 4) of
+// one workgroup, one replica per workgroup.  For large lattices with few
+// replicas (the L = 256 group of a batched run has 256), one wave per replica
+// leaves three of each CU's four SIMDs idle and the round latency exposed.
+// Slot t = 64 k + lane of wave k evaluates the attempt pair (2t, 2t+1) of the
+// round, exactly as sweep_kernel does with S = 64 W, so the trajectory is the
+// same sequential one.  Conflicts inside a wave are found as in sweep_kernel;
+// across waves, an attempt conflicts with every accepted attempt of an earlier
+// wave whose site its stencil holds.  A round has three barriers: after the
+// waves publish their accepted sites in LDS, after each publishes its first
+// invalid attempt (the round commits everything before the earliest), and
+// after the flips, so that the next round's reads see them.
+constexpr int kLdsWaveList = kLdsLattice;  // [W][128] accepted sites, [W] counts, [W] first invalid
+
+template <int RNG, int W, bool POW2>
+__global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
+    static_assert(W == 2 || W == 4, "two or four waves per replica");
+    constexpr int S = 64 * W;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
+    unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);
+    int *acc_lds = (int *)(smem + kLdsWaveList);
+    int *nacc_lds = acc_lds + W * 128;
+    int *finv_lds = nacc_lds + W;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int r = blockIdx.x;  // grid = R workgroups
+    // uniform over the workgroup
+    const bool active = !(a.freeze && a.fate[r] != -1);
+
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+        if (tid == i) thr_lds[i] = a.thr[i];
+    if (tid == 0) cnt_lds[0] = 0u;
+    __syncthreads();
+
+    Lattice<LAT_HBM> lat;
+    lat.base = (unsigned char *)a.lattice + (size_t)r * a.stride;
+    lat.dw = nullptr;
+    lat.rl = 0;
+
+    const uint64_t a_begin = a.a_begin_dev ? *a.a_begin_dev : a.a_begin;
+    const uint32_t b = (uint32_t)(a_begin & 1ull);
+    const uint64_t P0 = a_begin >> 1;
+    const uint32_t e = (uint32_t)a.n_attempts + b;
+    uint32_t l0 = active ? b : e;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const Geometry<POW2> geo{a.L, a.N, a.lg};
+    const uint32_t *drw = nullptr;
+    if constexpr (RNG == RNG_DRAWS) drw = a.draws + (size_t)r * (3 * a.n_attempts);
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+
+    while (l0 < e) {
+        const uint32_t q0 = l0 >> 1;
+        const uint32_t q = q0 + (uint32_t)tid;
+        const uint32_t lA = 2 * q, lB = lA + 1;
+        const bool enA = (lA >= l0) && (lA < e);
+        const bool enB = (lB >= l0) && (lB < e);
+
+        Stencil sA, sB;
+        uint32_t uaA, uaB;
+        if constexpr (RNG == RNG_PHILOX) {
+            const uint64_t pair = P0 + q;
+            uint32_t c0 = (uint32_t)pair, c1 = (uint32_t)(pair >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
+            philox4x32_10(c0, c1, c2, c3, k0, k1);
+            sA = geo.philox(c0);
+            uaA = c1;
+            sB = geo.philox(c2);
+            uaB = c3;
+        } else {
+            uint32_t u0 = 0, u1 = 0, u2 = 0, v0 = 0, v1 = 0, v2 = 0;
+            if (enA) {
+                const uint32_t *d = drw + 3 * (size_t)(lA - b);
+                u0 = d[0]; u1 = d[1]; u2 = d[2];
+            }
+            if (enB) {
+                const uint32_t *d = drw + 3 * (size_t)(lB - b);
+                v0 = d[0]; v1 = d[1]; v2 = d[2];
+            }
+            sA = geo.mt(u0, u1);
+            uaA = u2;
+            sB = geo.mt(v0, v1);
+            uaB = v2;
+        }
+        const Attempt A = evaluate<LAT_HBM>(lat, sA, uaA, enA, thr_lds);
+        const Attempt B = evaluate<LAT_HBM>(lat, sB, uaB, enB, thr_lds);
+
+        // conflicts inside the wave (the whole wave is one group)
+        const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
+        uint64_t cA = 0ull, cB = __ballot(A.acc && in_stencil(A.st.c, B.st));
+        uint64_t m = mA | mB;
+        while (m) {
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint64_t later = ~((2ull << i) - 1ull);  // i = 63: later = 0
+            if ((mA >> i) & 1ull) {
+                const int x = __builtin_amdgcn_readlane(A.st.c, i);
+                cA |= ballot_in_stencil(x, A.st) & later;
+                cB |= ballot_in_stencil(x, B.st) & later;
+            }
+            if ((mB >> i) & 1ull) {
+                const int x = __builtin_amdgcn_readlane(B.st.c, i);
+                cA |= ballot_in_stencil(x, A.st) & later;
+                cB |= ballot_in_stencil(x, B.st) & later;
+            }
+        }
+        // publish this wave's accepted sites (at most 128)
+        const int pos = (int)(__popcll(mA & lt_mask) + __popcll(mB & lt_mask));
+        if (A.acc) acc_lds[wave * 128 + pos] = A.st.c;
+        if (B.acc) acc_lds[wave * 128 + pos + (A.acc ? 1 : 0)] = B.st.c;
+        if (lane == 0) nacc_lds[wave] = (int)(__popcll(mA) + __popcll(mB));
+        __syncthreads();
+        // every attempt of this wave comes after every attempt of earlier
+        // waves; their lists are read into registers once (lane i holds
+        // entries i and 64 + i), then walked with readlane
+        for (int k = 0; k < wave; ++k) {
+            const int n = __builtin_amdgcn_readfirstlane(nacc_lds[k]);
+            const int v0 = acc_lds[k * 128 + lane];
+            const int v1 = acc_lds[k * 128 + 64 + lane];
+            const int n0 = n < 64 ? n : 64;
+            for (int i = 0; i < n0; ++i) {
+                const int x = __builtin_amdgcn_readlane(v0, i);
+                cA |= ballot_in_stencil(x, A.st);
+                cB |= ballot_in_stencil(x, B.st);
+            }
+            for (int i = 64; i < n; ++i) {
+                const int x = __builtin_amdgcn_readlane(v1, i - 64);
+                cA |= ballot_in_stencil(x, A.st);
+                cB |= ballot_in_stencil(x, B.st);
+            }
+        }
+        // this wave's first invalid attempt (local index 2 lane + (B ? 1 : 0); 128 = none)
+        const bool validA = (((cA | cB) & lt_mask) == 0ull) && !((cA >> lane) & 1ull);
+        const bool validB = validA && !((cB >> lane) & 1ull);
+        const uint64_t iA = __ballot(enA && !validA), iB = __ballot(enB && !validB);
+        const uint64_t x = iA | iB;
+        if (lane == 0) {
+            int f = 128;
+            if (x != 0ull) {
+                const int j = __builtin_ctzll(x);
+                f = 2 * j + (((iA >> j) & 1ull) ? 0 : 1);
+            }
+            finv_lds[wave] = f;
+        }
+        __syncthreads();
+        // the round's first invalid attempt, as an offset from attempt 2 q0
+        int F = 2 * S;
+        for (int k = W - 1; k >= 0; --k) {
+            const int f = __builtin_amdgcn_readfirstlane(finv_lds[k]);
+            if (f < 128) F = 128 * k + f;
+        }
+        const int tA = 128 * wave + 2 * lane;
+        if (A.acc && tA < F) lat.flip(A.st.c, A.s01);
+        if (B.acc && tA + 1 < F) lat.flip(B.st.c, B.s01);
+        if (F < 2 * S) {
+            l0 = 2 * q0 + (uint32_t)F;
+        } else {
+            const uint32_t nxt = 2 * (q0 + S);
+            l0 = nxt < e ? nxt : e;
+        }
+        // flips visible to every wave before the next round's reads; the
+        // lists are rewritten only after this barrier
+        __syncthreads();
+    }
+
+    // ---------------------------------------------------------- epilogue
+    __syncthreads();
+    unsigned c = 0;
+    const unsigned *w = (const unsigned *)lat.base;
+    for (int i = tid; i < a.stride / 4; i += S)
+        c += __popc(__hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0x01010101u);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (lane == 0) atomicAdd(&cnt_lds[0], c);
+    __syncthreads();
+    if (wave == 0) finish_replica(a, tid == 0, r, cnt_lds[0]);
+}
+
 // ------------------------------------------------------------ layout kernels
 // int32 +-1 grids [R][N] <-> native layouts.  1-D grid-stride loops over
 // (replica or group, element) so that any R fits the launch limits.

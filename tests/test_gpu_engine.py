@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 
 BETA, H = 0.54, 0.07
 # (method, slots per replica S, second tuning value: waves for 0/1, word split for 2)
+# Method 0 with 128 / 256 slots spreads a replica over 2 / 4 waves (sweep_hbm_waves).
 CASES = ([(2, s, 1) for s in (1, 2, 4, 8, 16)] + [(2, s, 2) for s in (1, 4, 16, 32)] +
-         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)])
+         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4)])
 
 
 def _start(kind, R, L, seed):
@@ -99,6 +100,22 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
     assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
+@pytest.mark.parametrize("S", [128, 256])
+def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
+    """A replica spread over 2 or 4 waves at the batched run's L = 256 shape,
+    from all -1 at the reference's (beta, h) and from a random start at
+    beta = 0.3 (about half the attempts accepted)."""
+    G, O = engine_lib, oracle
+    L, R, seed = 256, 6, 4242
+    for start, beta in (("down", BETA), ("random", 0.3)):
+        print(f"[L256 waves] S={S} start={start}", flush=True)
+        g0 = _start(start, R, L, seed)
+        got, mag = _run_engine(G, L, R, 0, S, g0, seed, False, beta, H, [1, 2], W=S // 64)
+        ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
+
+
 @pytest.mark.parametrize("L", [8, 16, 33, 64])
 def test_largest_cluster(gpu, engine_lib, oracle, L):
     G, O = engine_lib, oracle
@@ -119,8 +136,8 @@ def test_largest_cluster(gpu, engine_lib, oracle, L):
     assert np.array_equal(got_dn, ref_dn)
 
 
-@pytest.mark.parametrize("method,W", [(1, 0), (2, 0), (2, 2)])
-def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method, W):
+@pytest.mark.parametrize("method,S,W", [(1, 0, 0), (2, 0, 0), (2, 0, 2), (0, 256, 4)])
+def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method, S, W):
     """gasp_engine_set_inputs replicates input grids exactly as uploading the
     replicated grids; freezing decided replicas changes no fate at any check,
     and no undecided replica's trajectory."""
@@ -129,9 +146,9 @@ def test_inputs_and_freeze_leave_fates_unchanged(gpu, engine_lib, method, W):
     R = nin * per
     inputs = np.stack([np.where(np.add.outer(np.arange(L), np.arange(L)) % (k + 3) == 0, 1, -1)
                        for k in range(nin)]).astype(np.int32)
-    print(f"[freeze] method={method} W={W}", flush=True)
+    print(f"[freeze] method={method} S={S} W={W}", flush=True)
     runs = []
-    G.set_tuning(0, W)
+    G.set_tuning(S, W)
     for freeze in (False, True):
         rng = G.RngState(R, 99)
         with G.Engine(L, R, method, rng, 0.54, 0.07) as e:
