@@ -264,6 +264,13 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
     if ref is None:
         return port
     ref["port"] = port
+    try:
+        r1 = subprocess.run([sys.executable, "-m", "oracle.refbench", "--cores", "1", "--seconds", "4", "--L",
+                             str(L)], cwd=ROOT, capture_output=True, text=True, timeout=120)
+        if r1.returncode == 0:
+            ref["one_core"] = json.loads(r1.stdout.strip().splitlines()[-1])
+    except Exception:
+        pass
     return ref
 
 
