@@ -3,7 +3,9 @@
 // One sweep kernel template covers the three lattice layouts of the reference's
 // gpu_method 0/1/2 (src/mc_gpu.cu:88-151, 173-275, 279-434), the two random
 // streams (counter-based Philox4x32-10, or a pre-generated per-replica MT19937
-// draw stream that reproduces src/mc_cpu.c:41-89), and a speculation width S.
+// draw stream that reproduces src/mc_cpu.c:41-89), and a speculation width S
+// up to one wave; sweep_hbm_waves extends layout 0 to a replica spread over
+// two or four waves of a workgroup (S = 128, 256).
 //
 // Dynamics (identical in every layout).  Attempt a of a replica picks a site,
 // reads the spin s and its four periodic neighbours, forms
