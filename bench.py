@@ -392,6 +392,8 @@ def main() -> int:
             out["roofline"]["valu"] = {"instructions_per_attempt": vpa, "achieved": kern_rate * vpa,
                                        "peak": VALU_PEAK, "unit": "wave64 instr/s",
                                        "frac": kern_rate * vpa / VALU_PEAK}
+            # the binding resource is the higher of the two fractions
+            out["roofline"]["limiter"] = "valu" if kern_rate * vpa / VALU_PEAK > achieved / peak else "lds"
     G.set_tuning(0, 0)
     if not args.no_committor:
         out["committor"] = committor_leg(G)
