@@ -13,9 +13,12 @@ is fitted to 1 - exp(-k (t - t0)).
 
 Prints one JSON line.  `value` is spin-flip attempts per second over the timed
 steps (wall clock, HIP-synchronised on both sides).  `roofline` is for the
-sweep kernel, timed with HIP events on the engine's stream.  `cpu_baseline`
-times the oracle's restatement of the reference CPU sweep (src/mc_cpu.c, per
-replica MT19937 streams) on the host cores over a bounded sample.
+sweep kernel, timed with HIP events on the engine's stream: LDS bytes per the
+algorithmic figure, with the VALU issue rate beside it and the limiting one
+named.  `cpu_baseline` times the reference's own CPU sweep (src/mc_cpu.c,
+compiled from its sources into oracle/_ref) on up to 16 host cores over a
+bounded sample, with one core and the oracle's restatement reported beside it.
+The committor, surrogate and batched legs follow the throughput leg.
 """
 from __future__ import annotations
 
@@ -138,6 +141,8 @@ def surrogate_leg(G, shooting: dict, n_inputs: int = 512, fallback_reps: int = 1
         "seconds": secs, "predict_seconds": tm["predict_seconds"], "fallback_seconds": tm["fallback_seconds"],
         "speedup_vs_shooting": shooting["gpu_seconds"] / secs,
         "reference": "pB of the shooting leg (1024 replicas per input)",
+        # reliability of the network alone on these inputs: mean shooting pB per bin of prediction
+        "calibration": SG.calibration(tm["pred"][ok], ref[ok], bins=5),
         "training": {k: float(blob[k]) for k in ("val_mae", "n_train") if k in blob.files},
     }
 

@@ -196,9 +196,9 @@ def calibration(pred, label, bins=10):
 
 def estimate(grids, params, *, lo=0.2, hi=0.8, reps=128, seed=303, max_sweeps=20000):
     """Surrogate pB for all grids; shooting (`reps` replicas each) for
-    predictions in [lo, hi].  Returns estimates, the fallback mask, and
-    timings.  128 replicas bound the shooting standard error by
-    sqrt(0.25 / 128) = 0.044."""
+    predictions in [lo, hi].  Returns estimates, the fallback mask, and a dict
+    of timings and the network's raw predictions ("pred").  128 replicas bound
+    the shooting standard error by sqrt(0.25 / 128) = 0.044."""
     L = grids.shape[1]
     t0 = time.perf_counter()
     net = Surrogate(L, 4096)
@@ -213,7 +213,7 @@ def estimate(grids, params, *, lo=0.2, hi=0.8, reps=128, seed=303, max_sweeps=20
         pb, _, _, t_shoot = label_by_shooting(grids[unsure], reps=reps, seed=seed, max_sweeps=max_sweeps)
         est[unsure] = pb
     return est, unsure, {"predict_seconds": t_pred, "fallback_seconds": t_shoot,
-                         "seconds": time.perf_counter() - t0}
+                         "seconds": time.perf_counter() - t0, "pred": pred}
 
 
 FWD_FLOP = 2.29e8  # per L=64 grid: conv1 2.36 MFLOP + 3 x 75.5 MFLOP (32 -> 32, 3x3)
