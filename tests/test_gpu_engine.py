@@ -221,3 +221,48 @@ def test_gridstates_record_body(gpu, engine_lib, tmp_path, method, L):
     assert lib.write_ising_grids(L, R, full.ctypes.data_as(C.POINTER(C.c_int)), 0, None, None,
                                  b"magnetisation", -0.9, 0.9, fn) == 0
     assert open(fn, "rb").read()[12:] == body.tobytes()
+
+
+@pytest.mark.parametrize("method,S,W", [(2, 1, 1), (2, 16, 1), (2, 32, 2), (1, 4, 1), (0, 4, 1), (0, 128, 2)])
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_tiny_lattices(gpu, engine_lib, oracle, method, S, W, L):
+    """L = 2..5: the up and down (left and right) neighbours coincide at L = 2,
+    and every attempt of a round is a conflict-filter candidate.  Both random
+    streams, random starts at beta = 0.3."""
+    G, O = engine_lib, oracle
+    R, seed, beta = 70, 31337, 0.3
+    print(f"[tiny] method={method} S={S} W={W} L={L}", flush=True)
+    g0 = _start("random", R, L, seed)
+    got, _ = _run_engine(G, L, R, method, S, g0, seed, True, beta, H, [7, 5], W)
+    assert np.array_equal(got, O.sweep_mt_per_replica(L, g0.copy(), seed, 12, beta, H))
+    got, _ = _run_engine(G, L, R, method, S, g0, seed, False, beta, H, [7, 5], W)
+    assert np.array_equal(got, O.sweep_philox(L, g0.copy(), seed, 0, 12 * L * L, beta, H))
+
+
+@pytest.mark.parametrize("L,S,W", [(128, 16, 1), (142, 16, 1), (143, 32, 2)])
+def test_method2_largest_lattices(gpu, engine_lib, oracle, L, S, W):
+    """The largest lattices of the multispin layouts: whole 64-bit words up to
+    L = 142 (161 KB of LDS per workgroup), split words beyond; a partial last
+    group (R = 70)."""
+    G, O = engine_lib, oracle
+    R, seed = 70, 5150
+    print(f"[large] L={L} S={S} W={W}", flush=True)
+    g0 = _start("random", R, L, seed)
+    got, mag = _run_engine(G, L, R, 2, S, g0, seed, False, 0.3, H, [1, 1], W)
+    ref = O.sweep_philox(L, g0.copy(), seed, 0, 2 * L * L, 0.3, H)
+    assert np.array_equal(got, ref)
+
+
+def test_method2_whole_words_refused_past_L142(gpu, engine_lib):
+    """At L = 143 one group's 64-bit words exceed the 160 KB of LDS per
+    workgroup: the engine reports it instead of launching."""
+    G = engine_lib
+    print("[large] L=143 whole words", flush=True)
+    G.set_tuning(16, 1)
+    try:
+        rng = G.RngState(64, 1)
+        with pytest.raises(G.GaspError):
+            G.Engine(143, 64, 2, rng, BETA, H)
+        rng.free()
+    finally:
+        G.set_tuning(0, 0)
