@@ -371,6 +371,10 @@ struct gasp_engine {
     int *d_tmp32 = nullptr;
     size_t tmp32_bytes = 0;
     int freeze = 0;          // skip replicas whose fate is decided
+    int S0 = 1;              // slots per replica chosen at creation
+    bool adapt = false;      // raise the slots as replicas are decided (see engine_adapt_slots)
+    int decided = 0;         // decided replicas at the last counts read (or set_fates)
+    int ncu = 256;
     uint32_t *d_draws = nullptr;
     uint64_t draws_cap = 0;  // attempts per replica that fit in d_draws
     hipStream_t stream = nullptr;
@@ -443,8 +447,11 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     }
 
     choose_tuning(method, L, R, &e->S, &e->W);
+    e->S0 = e->S;
+    e->adapt = method == 2 && g_tune_slots <= 0;
     e->stride = layout_stride(method, e->N);
     DeviceInfo di = device_info();
+    if (di.ok) e->ncu = di.ncu;
     e->lds = lds_bytes_for(method, e->N, e->S, e->W);
     if (method == 2) {
         const int groups = (R + 63) / 64;
@@ -526,8 +533,35 @@ static int engine_sync(gasp_engine *e) {
     return engine_collect_events(e);
 }
 
+// An engine that freezes decided replicas (gasp_engine_set_freeze) raises its
+// slots per replica as replicas are decided, keeping about 16 waves per CU of
+// lanes on the undecided ones: a wave then holds fewer replicas, waves whose
+// replicas are all decided exit at once, and each undecided replica gets more
+// lanes.  Trajectories do not depend on the slots, so no result changes.
+// Method 2 only (its LDS does not depend on the slots); off when
+// gasp_set_tuning fixed the slots at creation.
+static int engine_adapt_slots(gasp_engine *e) {
+    if (!e->adapt || !e->freeze) return GASP_OK;
+    const long long target = 16LL * e->ncu * 64;
+    const long long active = std::max(1, e->R - e->decided);
+    const int smax = e->W == 2 ? 32 : 16;
+    const int S = std::max(std::min(pow2_ceil((target + active - 1) / active), smax), e->S0);
+    if (S == e->S) return GASP_OK;
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, S, e->W, is_pow2(e->L));
+    if (!fn) return GASP_OK;
+    if (e->lds > 65536)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds));
+    e->S = S;
+    e->block = e->rpw * S;
+    return GASP_OK;
+}
+
 static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *draws, int check,
                         double dn, double up, bool capture = false) {
+    if (!capture) {
+        const int rc = engine_adapt_slots(e);
+        if (rc) return rc;
+    }
     SweepArgs a{};
     a.lattice = e->d_lat;
     a.L = e->L;
@@ -892,12 +926,16 @@ extern "C" int gasp_engine_get_magnetisation(gasp_engine *e, float *host_out) {
 extern "C" int gasp_engine_get_counts(gasp_engine *e, int *counts3) {
     if (!e || !counts3) return set_error(GASP_ERR_ARG, "gasp_engine_get_counts: NULL argument");
     HIP_TRY(hipMemcpyAsync(counts3, e->d_counts, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    return engine_sync(e);
+    const int rc = engine_sync(e);
+    if (!rc) e->decided = counts3[1] + counts3[2];
+    return rc;
 }
 
 extern "C" int gasp_engine_set_fates(gasp_engine *e, const int *host_fates) {
     if (!e || !host_fates) return set_error(GASP_ERR_ARG, "gasp_engine_set_fates: NULL argument");
     HIP_TRY(hipMemcpyAsync(e->d_fate, host_fates, (size_t)e->R * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    e->decided = 0;
+    for (int r = 0; r < e->R; ++r) e->decided += host_fates[r] != -1;
     return engine_sync(e);
 }
 

@@ -211,7 +211,10 @@ int gasp_engine_set_grids(gasp_engine *e, const int *grids, int on_device);
 int gasp_engine_set_inputs(gasp_engine *e, const int *inputs, int ninputs);
 /* 1: replicas whose fate is decided stop sweeping (their state no longer
  * changes); pB and fates are unaffected.  Default 0 (the reference keeps
- * sweeping them). */
+ * sweeping them).  A freezing method-2 engine raises its slots per replica as
+ * replicas are decided (counted at gasp_engine_get_counts and
+ * gasp_engine_set_fates) unless gasp_set_tuning fixed the slots; trajectories
+ * do not depend on the slots. */
 int gasp_engine_set_freeze(gasp_engine *e, int freeze);
 int gasp_engine_get_grids(gasp_engine *e, int *grids, int on_device);
 /* Grids of the replicas listed in replicas[0..n) as int8 +-1, [n][L*L], in

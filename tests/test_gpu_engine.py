@@ -266,3 +266,42 @@ def test_method2_whole_words_refused_past_L142(gpu, engine_lib):
         rng.free()
     finally:
         G.set_tuning(0, 0)
+
+
+@pytest.mark.parametrize("W", [1, 2])
+def test_adaptive_slots_keep_trajectories(gpu, engine_lib, W):
+    """A freezing engine raises its slots per replica as replicas are decided
+    (here from 1 slot at 2^18 replicas).  Its fates at every check and its
+    final grids equal those of an engine held at one slot by gasp_set_tuning."""
+    G = engine_lib
+    L, nin, per = 16, 64, 4096
+    R = nin * per
+    inputs = np.stack([np.where(np.add.outer(np.arange(L), np.arange(L)) % (k % 7 + 3) == 0, 1, -1)
+                       for k in range(nin)]).astype(np.int32)
+    print(f"[adaptive] R={R} W={W}", flush=True)
+    runs = []
+    for slots in (1, 0):  # fixed at one slot, then automatic (adaptive)
+        G.set_tuning(slots, W)
+        try:
+            rng = G.RngState(R, 4321)
+            with G.Engine(L, R, 2, rng, 0.54, 0.07) as e:
+                e.set_inputs(inputs)
+                e.set_freeze(True)
+                e.set_fates(np.full(R, -1, dtype=np.int32))
+                e.sweep(0, G.CHECK_FATES, -0.8, 0.6)
+                e.counts()
+                hist = []
+                for _ in range(40):
+                    e.sweep(5, G.CHECK_FATES, -0.8, 0.6)
+                    hist.append((e.counts().copy(), e.fates()))
+                grids = e.get_grids()
+            rng.free()
+        finally:
+            G.set_tuning(0, 0)
+        runs.append((hist, grids))
+    (h0, g0), (h1, g1) = runs
+    for (c0, f0), (c1, f1) in zip(h0, h1):
+        assert np.array_equal(c0, c1) and np.array_equal(f0, f1)
+    assert np.array_equal(g0, g1)
+    # most replicas decided, so the adaptive engine ran several slot counts
+    assert (h1[-1][1] != -1).mean() > 0.6
