@@ -284,6 +284,11 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         }
         // several waves per replica only where conflicts are rare
         if (method == 0 && N < 16384) S = std::min(S, 64);
+        // With replicas to spare, 4 slots beat 1: at L=64 with 2^19 replicas
+        // a launch runs at 3.46e11 attempts/s with 4 slots (256-lane
+        // workgroups, four resident per CU) against 2.61e11 with one slot
+        // (64-lane workgroups limited to four per CU by their 32 KB of LDS).
+        if (method == 2) S = std::max(S, std::min(4, Smax));
     }
     if (method == 0 && S > 64) W = S / 64;  // one replica per workgroup
     if (method == 1) {

@@ -271,7 +271,7 @@ def test_method2_whole_words_refused_past_L142(gpu, engine_lib):
 @pytest.mark.parametrize("W", [1, 2])
 def test_adaptive_slots_keep_trajectories(gpu, engine_lib, W):
     """A freezing engine raises its slots per replica as replicas are decided
-    (here from 1 slot at 2^18 replicas).  Its fates at every check and its
+    (here from 4 slots at 2^18 replicas).  Its fates at every check and its
     final grids equal those of an engine held at one slot by gasp_set_tuning."""
     G = engine_lib
     L, nin, per = 16, 64, 4096
