@@ -289,6 +289,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         // workgroups, four resident per CU) against 2.61e11 with one slot
         // (64-lane workgroups limited to four per CU by their 32 KB of LDS).
         if (method == 2) S = std::max(S, std::min(4, Smax));
+        // Method 1 likewise: 8 slots ran at 3.34e11 against 2.44e11 with one.
+        if (method == 1) S = std::max(S, 8);
     }
     if (method == 0 && S > 64) W = S / 64;  // one replica per workgroup
     if (method == 1) {
