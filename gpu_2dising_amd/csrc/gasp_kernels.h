@@ -282,6 +282,21 @@ __device__ __forceinline__ void group_excl_or(uint32_t (&w)[NW], int slot) {
 
 __device__ __forceinline__ uint32_t fold32(uint64_t m) { return (uint32_t)m | (uint32_t)(m >> 32); }
 
+// Lanes of the S-lane groups (aligned, S < 64) that hold at least one set bit
+// of the wave mask x: OR each group into its first lane, keep those bits,
+// and spread each back over its group ((b << S) - b sets bits p .. p+S-1 for
+// a start bit p; the top group wraps to -2^p, also bits p .. 63).
+template <int S>
+__device__ __forceinline__ uint64_t group_span(uint64_t x) {
+#pragma unroll
+    for (int k = S / 2; k >= 1; k >>= 1) x |= x >> k;
+    uint64_t starts = 0ull;
+#pragma unroll
+    for (int p = 0; p < 64; p += S) starts |= 1ull << p;
+    const uint64_t b = x & starts;
+    return (b << S) - b;
+}
+
 // Per-replica epilogue: magnetisation and the threshold check.  Must be
 // called by a whole wave (ballots); `own` marks the lane reporting replica r.
 __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int r,
@@ -485,7 +500,11 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                                     ((w[4] >> (pA[2] & 31)) & (w[5] >> (pA[3] & 31)) & 1u);
                     const bool sB = ((er >> (pB[0] & 63)) & (ek >> (pB[1] & 63)) & 1ull) &&
                                     ((w[4] >> (pB[2] & 31)) & (w[5] >> (pB[3] & 31)) & 1u);
-                    if (__ballot(sA || sB) == 0ull) m = 0ull;
+                    // the exact loop then visits only the accepted attempts
+                    // of groups that hold a candidate (a group without one
+                    // has no conflict)
+                    const uint64_t cand = __ballot(sA || sB);
+                    m = cand ? (m & group_span<S>(cand)) : 0ull;
                 }
             }
             while (m) {
