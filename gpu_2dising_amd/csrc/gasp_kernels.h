@@ -41,11 +41,11 @@ enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
 // LDS header shared by all layouts: 10 thresholds (80 B, padded to 128),
 // 64 per-replica counters and the 64 row/column neighbourhood masks of the
-// conflict filter.
+// conflict filter plus a zero entry.
 constexpr int kLdsThr = 0;
 constexpr int kLdsCnt = 128;
 constexpr int kLdsNbr = 384;
-constexpr int kLdsLattice = 896;
+constexpr int kLdsLattice = 912;  // 384 + 65 masks of 8 bytes, rounded up to 16
 
 struct SweepArgs {
     void *lattice;           // native layout (see engine)
@@ -371,7 +371,8 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     for (int i = 0; i < 10; ++i)
         if (tid == i) thr_lds[i] = a.thr[i];
     if (tid < 64) cnt_lds[tid] = 0u;
-    for (int t = tid; t < 64; t += blockDim.x) nbr_lds[t] = nbr_mask(t, a.L);
+    // entry 64 is zero: the mask of an attempt that was not accepted
+    for (int t = tid; t <= 64; t += blockDim.x) nbr_lds[t] = t < 64 ? nbr_mask(t, a.L) : 0ull;
     // the row / column filter needs exact (L <= 64) or aligned (64 | L) folding
     const bool use_filter = POW2 || a.L <= 64;
     if constexpr (LAT == LAT_MULTISPIN) {
@@ -488,8 +489,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                     uint64_t mAk[4], mBk[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        mAk[k] = A.acc ? nbr_lds[pA[k] & 63] : 0ull;
-                        mBk[k] = B.acc ? nbr_lds[pB[k] & 63] : 0ull;
+                        // not accepted: the zero entry (a select, not a masked read)
+                        mAk[k] = nbr_lds[A.acc ? (pA[k] & 63) : 64];
+                        mBk[k] = nbr_lds[B.acc ? (pB[k] & 63) : 64];
                     }
                     const uint64_t rm = mAk[0] | mBk[0], km = mAk[1] | mBk[1];
                     uint32_t w[6] = {(uint32_t)rm, (uint32_t)(rm >> 32), (uint32_t)km, (uint32_t)(km >> 32),
