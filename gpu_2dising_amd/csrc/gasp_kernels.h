@@ -282,6 +282,13 @@ __device__ __forceinline__ void group_excl_or(uint32_t (&w)[NW], int slot) {
 
 __device__ __forceinline__ uint32_t fold32(uint64_t m) { return (uint32_t)m | (uint32_t)(m >> 32); }
 
+// The filter tests rows and columns only: once the exact loop visits only the
+// groups that hold a candidate, the diagonal projections (two more masks per
+// attempt and two more scanned dwords) cost more than the extra candidates
+// they remove.  With them the candidate test is exact for one earlier
+// accepted attempt; without them it also admits the four diagonal neighbours.
+constexpr bool kFilterDiagonals = false;
+
 // Lanes of the S-lane groups (aligned, S < 64) that hold at least one set bit
 // of the wave mask x: OR each group into its first lane, keep those bits,
 // and spread each back over its group ((b << S) - b sets bits p .. p+S-1 for
@@ -475,33 +482,39 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                 // unchanged otherwise).  Rows and columns use 64-bit masks,
                 // the two diagonals masks folded to 32 bits.
                 if (use_filter && m != 0ull) {
+                    constexpr int NP = kFilterDiagonals ? 4 : 2;
                     int pA[4], pB[4];  // row, col, (row + col) mod L, (row - col) mod L
                     pA[0] = A.st.row, pA[1] = A.st.col, pB[0] = B.st.row, pB[1] = B.st.col;
-                    if constexpr (POW2) {
-                        pA[2] = (pA[0] + pA[1]) & (L - 1), pA[3] = (pA[0] - pA[1]) & (L - 1);
-                        pB[2] = (pB[0] + pB[1]) & (L - 1), pB[3] = (pB[0] - pB[1]) & (L - 1);
-                    } else {
-                        pA[2] = pA[0] + pA[1], pA[3] = pA[0] - pA[1];
-                        pB[2] = pB[0] + pB[1], pB[3] = pB[0] - pB[1];
-                        pA[2] -= pA[2] >= L ? L : 0, pA[3] += pA[3] < 0 ? L : 0;
-                        pB[2] -= pB[2] >= L ? L : 0, pB[3] += pB[3] < 0 ? L : 0;
+                    if constexpr (kFilterDiagonals) {
+                        if constexpr (POW2) {
+                            pA[2] = (pA[0] + pA[1]) & (L - 1), pA[3] = (pA[0] - pA[1]) & (L - 1);
+                            pB[2] = (pB[0] + pB[1]) & (L - 1), pB[3] = (pB[0] - pB[1]) & (L - 1);
+                        } else {
+                            pA[2] = pA[0] + pA[1], pA[3] = pA[0] - pA[1];
+                            pB[2] = pB[0] + pB[1], pB[3] = pB[0] - pB[1];
+                            pA[2] -= pA[2] >= L ? L : 0, pA[3] += pA[3] < 0 ? L : 0;
+                            pB[2] -= pB[2] >= L ? L : 0, pB[3] += pB[3] < 0 ? L : 0;
+                        }
                     }
                     uint64_t mAk[4], mBk[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                    for (int k = 0; k < NP; ++k) {
                         // not accepted: the zero entry (a select, not a masked read)
                         mAk[k] = nbr_lds[A.acc ? (pA[k] & 63) : 64];
                         mBk[k] = nbr_lds[B.acc ? (pB[k] & 63) : 64];
                     }
                     const uint64_t rm = mAk[0] | mBk[0], km = mAk[1] | mBk[1];
-                    uint32_t w[6] = {(uint32_t)rm, (uint32_t)(rm >> 32), (uint32_t)km, (uint32_t)(km >> 32),
-                                     fold32(mAk[2] | mBk[2]), fold32(mAk[3] | mBk[3])};
+                    uint32_t w[NP == 4 ? 6 : 4];
+                    w[0] = (uint32_t)rm, w[1] = (uint32_t)(rm >> 32), w[2] = (uint32_t)km, w[3] = (uint32_t)(km >> 32);
+                    if constexpr (kFilterDiagonals) w[4] = fold32(mAk[2] | mBk[2]), w[5] = fold32(mAk[3] | mBk[3]);
                     group_excl_or<S>(w, slot);
                     const uint64_t er = ((uint64_t)w[1] << 32) | w[0], ek = ((uint64_t)w[3] << 32) | w[2];
-                    const bool sA = ((er >> (pA[0] & 63)) & (ek >> (pA[1] & 63)) & 1ull) &&
-                                    ((w[4] >> (pA[2] & 31)) & (w[5] >> (pA[3] & 31)) & 1u);
-                    const bool sB = ((er >> (pB[0] & 63)) & (ek >> (pB[1] & 63)) & 1ull) &&
-                                    ((w[4] >> (pB[2] & 31)) & (w[5] >> (pB[3] & 31)) & 1u);
+                    bool sA = ((er >> (pA[0] & 63)) & (ek >> (pA[1] & 63)) & 1ull) != 0ull;
+                    bool sB = ((er >> (pB[0] & 63)) & (ek >> (pB[1] & 63)) & 1ull) != 0ull;
+                    if constexpr (kFilterDiagonals) {
+                        sA = sA && ((w[4] >> (pA[2] & 31)) & (w[5] >> (pA[3] & 31)) & 1u);
+                        sB = sB && ((w[4] >> (pB[2] & 31)) & (w[5] >> (pB[3] & 31)) & 1u);
+                    }
                     // the exact loop then visits only the accepted attempts
                     // of groups that hold a candidate (a group without one
                     // has no conflict)
