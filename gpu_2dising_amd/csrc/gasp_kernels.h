@@ -476,11 +476,11 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             uint64_t m = mA | mB;
             if constexpr (S <= 16) {
                 // Filter (see nbr_mask): union the projection masks of the
-                // accepted attempts of the earlier lanes of the group; if no
-                // attempt of the wave lies within 1 on all four projections,
-                // none conflicts and the exact loop is skipped (it runs
-                // unchanged otherwise).  Rows and columns use 64-bit masks,
-                // the two diagonals masks folded to 32 bits.
+                // accepted attempts of the earlier lanes of the group; an
+                // attempt within 1 on every projection tested (rows and
+                // columns as 64-bit masks; with kFilterDiagonals also the two
+                // diagonals folded to 32 bits) is a candidate.  Without a
+                // candidate in the wave the exact loop is skipped.
                 if (use_filter && m != 0ull) {
                     constexpr int NP = kFilterDiagonals ? 4 : 2;
                     int pA[4], pB[4];  // row, col, (row + col) mod L, (row - col) mod L
