@@ -342,7 +342,7 @@ def main() -> int:
     bpa = HBM_BYTES_PER_ATTEMPT if hbm else LDS_BYTES_PER_ATTEMPT
     peak = HBM_PEAK_GBS if hbm else LDS_PEAK_GBS
     achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
-    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: "LAT_MULTISPIN"}[args.method]
+    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: "LAT_MULTIHALF" if split == 2 else "LAT_MULTISPIN"}[args.method]
     sweeps = np.arange(1, args.steps + 1) * SWEEPS_PER_STEP
     k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
 

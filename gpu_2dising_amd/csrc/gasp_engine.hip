@@ -265,7 +265,12 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     const size_t lds_block = d.ok ? d.lds_block : 65536;
     const int N = L * L;
     int W;
-    if (method == 2) W = (g_tune_waves == 2 || g_tune_slots > 16) ? 2 : 1;
+    // Method 2 splits the 64-bit words (32 replicas per workgroup, a dword
+    // per site in LDS) unless whole words are asked for: at L=64 with 16384
+    // replicas and 16 slots a launch takes 21.5 ms split against 21.8 ms
+    // whole, with 2^19 replicas and 8 slots 4.44e11 attempts/s against 4.29e11,
+    // and split words fit lattices up to L=201 instead of 142.
+    if (method == 2) W = (g_tune_waves == 1 && g_tune_slots <= 16) ? 1 : 2;
     else W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
     // method 0 may spread a replica over 2 or 4 waves (sweep_hbm_waves)
     int Smax = (method == 2) ? (W == 2 ? 32 : 16) : (method == 0 ? 256 : 64);
@@ -284,11 +289,11 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         }
         // several waves per replica only where conflicts are rare
         if (method == 0 && N < 16384) S = std::min(S, 64);
-        // With replicas to spare, 4 slots beat 1: at L=64 with 2^19 replicas
-        // a launch runs at 3.46e11 attempts/s with 4 slots (256-lane
-        // workgroups, four resident per CU) against 2.61e11 with one slot
-        // (64-lane workgroups limited to four per CU by their 32 KB of LDS).
-        if (method == 2) S = std::max(S, std::min(4, Smax));
+        // With replicas to spare, several slots beat one: at L=64 with 2^19
+        // replicas one slot per replica (64-lane workgroups, four per CU for
+        // LDS) ran at 2.61e11 attempts/s; split words with 8 slots run at
+        // 4.44e11 (4 slots: 3.88e11, 16 slots: 4.09e11).
+        if (method == 2) S = std::max(S, std::min(8, Smax));
         // Method 1 likewise: 8 slots ran at 3.34e11 against 2.44e11 with one.
         if (method == 1) S = std::max(S, 8);
     }

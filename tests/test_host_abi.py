@@ -193,8 +193,8 @@ def test_default_launch_shapes(engine_lib):
     """Launch shapes chosen without the tuner (256 CUs): method 0 runs a whole
     wave per replica from N >= 4096, and 2 or 4 waves per replica when
     N >= 16384 and the replicas alone would not fill the device; method 2 keeps
-    16 slots per replica at L=64 with 16384 replicas and never goes below 4 (method
-    1: 8).  An explicit 128-slot
+    split words with 16 slots per replica at L=64 with 16384 replicas and never
+    goes below 8 slots (method 1: 8).  An explicit 128-slot
     setting for method 0 implies two waves."""
     G = engine_lib
     G.set_tuning(0, 0)
@@ -204,8 +204,9 @@ def test_default_launch_shapes(engine_lib):
         assert G.get_tuning(0, 256, 256) == (256, 4)
         assert G.get_tuning(0, 256, 2048) == (128, 2)
         assert G.get_tuning(0, 256, 16384) == (64, 1)
-        assert G.get_tuning(2, 64, 16384) == (16, 1)
-        assert G.get_tuning(2, 64, 524288) == (4, 1)  # at least 4 slots with replicas to spare
+        assert G.get_tuning(2, 64, 16384) == (16, 2)  # split words
+        assert G.get_tuning(2, 64, 524288) == (8, 2)  # at least 8 slots with replicas to spare
+        assert G.get_tuning(2, 160, 64) == (32, 2)  # whole words would not fit LDS at L = 160
         assert G.get_tuning(1, 64, 524288) == (8, 1)  # method 1: at least 8
         G.set_tuning(128, 0)
         assert G.get_tuning(0, 16, 96) == (128, 2)
