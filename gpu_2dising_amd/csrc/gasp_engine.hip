@@ -294,6 +294,10 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         // LDS) ran at 2.61e11 attempts/s; split words with 8 slots run at
         // 4.44e11 (4 slots: 3.88e11, 16 slots: 4.09e11).
         if (method == 2) S = std::max(S, std::min(8, Smax));
+        // ... and at most 16, the widest group the conflict filter covers
+        // (32 unfiltered slots made the surrogate's fallback shooting, 16256
+        // replicas, 50% slower); gasp_set_tuning can still ask for 32
+        if (method == 2) S = std::min(S, 16);
         // Method 1 likewise: 8 slots ran at 3.34e11 against 2.44e11 with one.
         if (method == 1) S = std::max(S, 8);
     }
@@ -556,7 +560,10 @@ static int engine_adapt_slots(gasp_engine *e) {
     if (!e->adapt || !e->freeze) return GASP_OK;
     const long long target = 16LL * e->ncu * 64;
     const long long active = std::max(1, e->R - e->decided);
-    const int smax = e->W == 2 ? 32 : 16;
+    // up to 16: the conflict filter covers groups of at most 16 lanes, and
+    // unfiltered 32-lane groups made the surrogate's fallback shooting 50%
+    // slower
+    const int smax = 16;
     const int S = std::max(std::min(pow2_ceil((target + active - 1) / active), smax), e->S0);
     if (S == e->S) return GASP_OK;
     SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, S, e->W, is_pow2(e->L));

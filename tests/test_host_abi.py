@@ -206,7 +206,7 @@ def test_default_launch_shapes(engine_lib):
         assert G.get_tuning(0, 256, 16384) == (64, 1)
         assert G.get_tuning(2, 64, 16384) == (16, 2)  # split words
         assert G.get_tuning(2, 64, 524288) == (8, 2)  # at least 8 slots with replicas to spare
-        assert G.get_tuning(2, 160, 64) == (32, 2)  # whole words would not fit LDS at L = 160
+        assert G.get_tuning(2, 160, 64) == (16, 2)  # split: whole words would not fit LDS at L = 160
         assert G.get_tuning(1, 64, 524288) == (8, 1)  # method 1: at least 8
         G.set_tuning(128, 0)
         assert G.get_tuning(0, 16, 96) == (128, 2)
