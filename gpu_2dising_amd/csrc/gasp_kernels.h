@@ -249,7 +249,7 @@ __device__ __forceinline__ uint32_t row_shr(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + K, 0xf, 0xf, true);
 }
 
-// OR of w[i] over the lanes of the group (S <= 16 consecutive, aligned lanes)
+// OR of w[i] over the lanes of the group (S consecutive, aligned lanes)
 // that precede this lane, for each of the NW dwords: a shift by one lane, then
 // an inclusive Hillis-Steele OR-scan (steps k < S) inside the 16-lane DPP row.
 template <int S, int K, int NW>
@@ -264,20 +264,42 @@ __device__ __forceinline__ void scan_step(uint32_t (&w)[NW], int slot) {
     }
 }
 
+// Groups of 32 or 64 lanes span DPP rows: after the scan inside each row,
+// row_bcast:15 ORs the total of row 0 (2) into every lane of row 1 (3), and
+// for 64-lane groups row_bcast:31 then ORs the total of rows 0-1 into rows 2
+// and 3.  `x` holds each lane's own values (the row totals are inclusive).
 template <int S, int NW>
 __device__ __forceinline__ void group_excl_or(uint32_t (&w)[NW], int slot) {
-    static_assert(S >= 2 && S <= 16, "DPP scan within a 16-lane row");
+    static_assert(S >= 2 && S <= 64, "2 to 64 lanes per group");
+    constexpr int SR = S < 16 ? S : 16;  // lanes of the group inside one row
+    uint32_t x[NW];
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
+        x[i] = w[i];
         w[i] = row_shr<1>(w[i]);
         if constexpr (S < 16) {
             if (slot == 0) w[i] = 0u;
         }
     }
-    scan_step<S, 1>(w, slot);
-    if constexpr (S > 2) scan_step<S, 2>(w, slot);
-    if constexpr (S > 4) scan_step<S, 4>(w, slot);
-    if constexpr (S > 8) scan_step<S, 8>(w, slot);
+    scan_step<SR, 1>(w, slot);
+    if constexpr (SR > 2) scan_step<SR, 2>(w, slot);
+    if constexpr (SR > 4) scan_step<SR, 4>(w, slot);
+    if constexpr (SR > 8) scan_step<SR, 8>(w, slot);
+    if constexpr (S >= 32) {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t tot = w[i] | x[i];  // inclusive within the row
+            // rows 1 and 3 receive lane 15 / lane 47; rows 0 and 2 keep 0
+            const uint32_t p = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)tot, 0x142, 0xa, 0xf, false);
+            uint32_t e = w[i] | p;
+            if constexpr (S == 64) {
+                // rows 2 and 3 receive lane 31, the total of rows 0 and 1
+                const uint32_t q = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(tot | p), 0x143, 0xc, 0xf, false);
+                e |= q;
+            }
+            w[i] = e;
+        }
+    }
 }
 
 __device__ __forceinline__ uint32_t fold32(uint64_t m) { return (uint32_t)m | (uint32_t)(m >> 32); }
@@ -295,6 +317,7 @@ constexpr bool kFilterDiagonals = false;
 // a start bit p; the top group wraps to -2^p, also bits p .. 63).
 template <int S>
 __device__ __forceinline__ uint64_t group_span(uint64_t x) {
+    if constexpr (S == 64) return x ? ~0ull : 0ull;  // one group: the whole wave
 #pragma unroll
     for (int k = S / 2; k >= 1; k >>= 1) x |= x >> k;
     uint64_t starts = 0ull;
@@ -474,7 +497,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
             uint64_t cA = 0ull, cB = __ballot(ownB);
             uint64_t m = mA | mB;
-            if constexpr (S <= 16) {
+            // The filter pays up to 32 lanes per group (a 32-slot split launch
+            // at L=64, 16384 replicas: 42.5 -> 37.9 ms); over a whole wave
+            // (method 0 at 64 slots) it cost more than it saved (39.1 -> 40.6).
+            if constexpr (S <= 32) {
                 // Filter (see nbr_mask): union the projection masks of the
                 // accepted attempts of the earlier lanes of the group; an
                 // attempt within 1 on every projection tested (rows and
