@@ -9,7 +9,10 @@ init_genrand seed, all processes start together, and the attempt rate is the
 total attempts over the slowest process's time.
 
   python3 -m oracle.refbench --cores 16 --seconds 12 --L 64
-prints one JSON object.
+prints one JSON object.  --driver instead times the reference's literal
+single-stream schedule, src/mc_cpu.c:106-308 mc_driver_cpu (one MT19937 stream
+over all replicas, checks every 100 sweeps), on config 1's shape (L = 32, 256
+replicas, beta = 0.54, h = 0.07, all -1) for a bounded number of sweeps.
 """
 from __future__ import annotations
 
@@ -69,13 +72,77 @@ def run(cores: int, seconds: float, L: int = 64, beta: float = 0.54, h: float = 
                       f"{nsweeps} sweeps, L={L}, beta={beta}, h={h}, all -1 start, {dt:.1f} s"}
 
 
+def _driver_child(nsweeps, q):
+    import ctypes as C
+    from oracle import oracle as O
+
+    # the driver prints a line per check: keep it off this process's stdout
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+
+    class McGrids(C.Structure):  # include/mc_cpu.h:5-9
+        _fields_ = [("L", C.c_int), ("ngrids", C.c_int), ("ising_grids", C.POINTER(C.c_int))]
+
+    class McSampler(C.Structure):  # include/mc_cpu.h:11-15
+        _fields_ = [("tot_nsweeps", C.c_int), ("mag_output_int", C.c_int), ("grid_output_int", C.c_int)]
+
+    class McFunction(C.Structure):  # include/mc_cpu.h:17-26
+        _fields_ = [("itask", C.c_int), ("dn_thr", C.c_double), ("up_thr", C.c_double), ("ninputs", C.c_int),
+                    ("initial_spin", C.c_int), ("cv", C.c_char_p), ("result", C.POINTER(C.c_float)),
+                    ("filename", C.c_char_p)]
+
+    outf_t = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.c_char_p, C.c_double, C.c_double, C.c_char_p)
+    R = O.ref()
+    R.mc_driver_cpu.argtypes = [McGrids, C.c_double, C.c_double, C.POINTER(C.c_int), McSampler, McFunction, outf_t]
+    R.mc_driver_cpu.restype = None
+    L, n, beta, h, mag = 32, 256, 0.54, 0.07, 100
+    grids = -np.ones(n * L * L, dtype=np.int32)
+    fates = np.full(n, -1, dtype=np.int32)
+    result = np.zeros(nsweeps // mag + 1, dtype=np.float32)
+    noop = outf_t(lambda *args: 0)
+    R.init_genrand(2894203475)
+    R.preComputeProbs_cpu(beta, h)
+    t0 = time.perf_counter()
+    R.mc_driver_cpu(McGrids(L, n, grids.ctypes.data_as(C.POINTER(C.c_int))), beta, h,
+                    fates.ctypes.data_as(C.POINTER(C.c_int)), McSampler(nsweeps, mag, 10 * nsweeps),
+                    McFunction(0, -0.9, 0.9, 1, -1, b"magnetisation", result.ctypes.data_as(C.POINTER(C.c_float)),
+                               b"gridstates.bin"), noop)
+    dt = time.perf_counter() - t0
+    # itask 0 stops at the first check where every replica has nucleated
+    # (src/mc_cpu.c:218-220): that check's sweep count is what ran
+    full = np.nonzero(result[: nsweeps // mag + 1] >= 1.0)[0]
+    done = int(full[0]) * mag if full.size else nsweeps
+    q.put((done * n * L * L, dt))
+
+
+def run_driver(nsweeps: int = 1000) -> dict:
+    """src/mc_cpu.c mc_driver_cpu on config 1's shape, in a forked child."""
+    from oracle import oracle as O
+
+    if O.ref() is None:
+        raise RuntimeError("oracle/_ref/libgasp_ref.so is not built")
+    ctx = mp.get_context("fork")
+    q = ctx.Queue()
+    p = ctx.Process(target=_driver_child, args=(nsweeps, q))
+    p.start()
+    attempts, dt = q.get()
+    p.join()
+    return {"value": attempts / dt, "unit": "attempts/s", "cores": 1, "kind": "reference",
+            "sample": f"src/mc_cpu.c mc_driver_cpu (compiled reference, one MT19937 stream over all replicas), "
+                      f"L=32, 256 replicas x {nsweeps} sweeps, checks every 100, beta=0.54, h=0.07, all -1, "
+                      f"{dt:.1f} s"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--cores", type=int, default=1)
     ap.add_argument("--seconds", type=float, default=12.0)
     ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--driver", action="store_true", help="time mc_driver_cpu on config 1's shape")
+    ap.add_argument("--sweeps", type=int, default=1000, help="--driver: sweeps")
     a = ap.parse_args()
-    print(json.dumps(run(a.cores, a.seconds, a.L)), flush=True)
+    print(json.dumps(run_driver(a.sweeps) if a.driver else run(a.cores, a.seconds, a.L)), flush=True)
     return 0
 
 
