@@ -221,7 +221,7 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
 
 static int pow2_ceil(long long x) {
     int p = 1;
-    while (p < x && p < 256) p <<= 1;
+    while (p < x && p < 512) p <<= 1;
     return p;
 }
 
@@ -273,7 +273,7 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     if (method == 2) W = (g_tune_waves == 1 && g_tune_slots <= 16) ? 1 : 2;
     else W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
     // method 0 may spread a replica over 2 or 4 waves (sweep_hbm_waves)
-    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : (method == 0 ? 256 : 64);
+    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : (method == 0 ? 512 : 64);
     int S;
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), Smax);
@@ -289,6 +289,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         }
         // several waves per replica only where conflicts are rare
         if (method == 0 && N < 16384) S = std::min(S, 64);
+        // eight waves per replica (512 slots) only when the tuner picks it
+        if (method == 0) S = std::min(S, 256);
         // With replicas to spare, several slots beat one: at L=64 with 2^19
         // replicas one slot per replica (64-lane workgroups, four per CU for
         // LDS) ran at 2.61e11 attempts/s; split words with 8 slots run at
@@ -345,6 +347,7 @@ static SweepFn pick_sweep(int S) {
     if constexpr (LAT == LAT_HBM) {
         if (S == 128) return sweep_hbm_waves<RNG, 2, P2>;
         if (S == 256) return sweep_hbm_waves<RNG, 4, P2>;
+        if (S == 512) return sweep_hbm_waves<RNG, 8, P2>;
     }
     return nullptr;
 }
@@ -1227,6 +1230,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
     if (method == 0) {
         cand.push_back({128, 2});
         cand.push_back({256, 4});
+        cand.push_back({512, 8});
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();

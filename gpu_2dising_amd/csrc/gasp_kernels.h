@@ -685,7 +685,7 @@ constexpr int kLdsWaveList = kLdsLattice;  // [W][128] accepted sites, [W] count
 
 template <int RNG, int W, bool POW2>
 __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
-    static_assert(W == 2 || W == 4, "two or four waves per replica");
+    static_assert(W == 2 || W == 4 || W == 8, "two, four or eight waves per replica");
     constexpr int S = 64 * W;
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);

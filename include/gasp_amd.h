@@ -177,8 +177,9 @@ void gasp_free(void *d_ptr);
  * split: 1 = a workgroup holds the 64 replicas of a 64-bit site word (slots up
  * to 16), 2 = it holds one 32-bit half (32 replicas, slots up to 32); 0 =
  * automatic, which splits.
- * Method 0 also takes 128 or 256 slots: one replica per workgroup, spread over
- * 2 or 4 waves (the waves value is then implied).  Other methods: up to 64.
+ * Method 0 also takes 128, 256 or 512 slots: one replica per workgroup, spread
+ * over 2, 4 or 8 waves (the waves value is then implied).  Other methods: up
+ * to 64.
  * Applies to subsequent runs. */
 void gasp_set_tuning(int slots_per_replica, int waves_per_group);
 void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,

@@ -17,9 +17,9 @@ pytestmark = pytest.mark.gpu
 
 BETA, H = 0.54, 0.07
 # (method, slots per replica S, second tuning value: waves for 0/1, word split for 2)
-# Method 0 with 128 / 256 slots spreads a replica over 2 / 4 waves (sweep_hbm_waves).
+# Method 0 with 128 / 256 / 512 slots spreads a replica over 2 / 4 / 8 waves (sweep_hbm_waves).
 CASES = ([(2, s, 1) for s in (1, 2, 4, 8, 16)] + [(2, s, 2) for s in (1, 4, 16, 32)] +
-         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4)])
+         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4), (0, 512, 8)])
 
 
 def _start(kind, R, L, seed):
@@ -100,7 +100,7 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
     assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
-@pytest.mark.parametrize("S", [128, 256])
+@pytest.mark.parametrize("S", [128, 256, 512])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
     """A replica spread over 2 or 4 waves at the batched run's L = 256 shape,
     from all -1 at the reference's (beta, h) and from a random start at
