@@ -227,7 +227,7 @@ static int pow2_ceil(long long x) {
 
 // Native layout strides.
 static int layout_stride(int method, int N) {
-    if (method == 2) return N + (N & 1);          // 64-bit words per 64-replica group
+    if (method == 2) return (N + 3) & ~3;         // dwords per plane, two planes per 64-replica group
     if (method == 1) return (N + 31) / 32;        // dwords per replica
     return ((N + 15) / 16) * 16;                  // bytes per replica
 }
@@ -675,7 +675,7 @@ static int engine_set_grids_impl(gasp_engine *e, const int *grids, int on_device
     const int N = e->N, R = e->R, per = R / ninputs;
     const dim3 grid(1024), blk(256);
     if (e->method == 2) {
-        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (uint64_t *)e->d_lat, R, N, e->stride, per);
+        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride, per);
     } else if (e->method == 1) {
         hipLaunchKernelGGL(pack_bits, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride, per);
     } else {
@@ -690,7 +690,7 @@ static int engine_unpack(gasp_engine *e, T *dst) {
     const int N = e->N, R = e->R;
     const dim3 grid(1024), blk(256);
     if (e->method == 2) {
-        hipLaunchKernelGGL(unpack_multispin<T>, grid, blk, 0, e->stream, (const uint64_t *)e->d_lat, dst, R, N,
+        hipLaunchKernelGGL(unpack_multispin<T>, grid, blk, 0, e->stream, (const unsigned *)e->d_lat, dst, R, N,
                            e->stride);
     } else if (e->method == 1) {
         hipLaunchKernelGGL(unpack_bits<T>, grid, blk, 0, e->stream, (const unsigned *)e->d_lat, dst, R, N, e->stride);

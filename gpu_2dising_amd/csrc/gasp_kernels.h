@@ -52,7 +52,7 @@ struct SweepArgs {
     int L, N, R;
     int lg;                  // log2(L) when L is a power of two
     int rpw;                 // replicas per workgroup (layouts 0/1)
-    int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: words per group
+    int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: dwords per plane (2 per group)
     uint64_t a_begin;        // per-replica attempt index of the first attempt
     const uint64_t *a_begin_dev;  // if set, a_begin is read from device memory (graph replays)
     uint64_t n_attempts;     // attempts per replica in this launch (< 2^31)
@@ -406,17 +406,22 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     // the row / column filter needs exact (L <= 64) or aligned (64 | L) folding
     const bool use_filter = POW2 || a.L <= 64;
     if constexpr (LAT == LAT_MULTISPIN) {
-        // a.stride words per group (N rounded up to even): 16-byte aligned
-        const uint4 *src = (const uint4 *)((const uint64_t *)a.lattice + (size_t)blockIdx.x * a.stride);
+        // HBM: the group's two dword planes (bits 0..31, then 32..63 of each
+        // site's word), a.stride dwords each (N rounded up to 4); LDS: one
+        // 64-bit word per site (dwords 2 site and 2 site + 1)
+        const uint4 *lo = (const uint4 *)((const unsigned *)a.lattice + (size_t)(2 * blockIdx.x) * a.stride);
+        const uint4 *hi = lo + a.stride / 4;
         uint4 *dst = (uint4 *)lat_lds;
-        const int n16 = a.stride / 2;
-        for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+        for (int i = tid; i < a.stride / 4; i += blockDim.x) {
+            const uint4 l = lo[i], h = hi[i];
+            dst[2 * i] = make_uint4(l.x, h.x, l.y, h.y);
+            dst[2 * i + 1] = make_uint4(l.z, h.z, l.w, h.w);
+        }
     } else if constexpr (LAT == LAT_MULTIHALF) {
-        // dword (2 site + half) of the group's words -> LDS dword `site`
-        const unsigned *src = (const unsigned *)((const uint64_t *)a.lattice + (size_t)(blockIdx.x >> 1) * a.stride) +
-                              (blockIdx.x & 1);
-        unsigned *dst = (unsigned *)lat_lds;
-        for (int i = tid; i < a.stride; i += blockDim.x) dst[i] = src[2 * i];
+        // workgroup 2g + h stages plane h of group g, which is plane blockIdx.x
+        const uint4 *src = (const uint4 *)((const unsigned *)a.lattice + (size_t)blockIdx.x * a.stride);
+        uint4 *dst = (uint4 *)lat_lds;
+        for (int i = tid; i < a.stride / 4; i += blockDim.x) dst[i] = src[i];
     } else if constexpr (LAT == LAT_BITS) {
         const int nrep = min(a.rpw, a.R - r0);
         const int ndw = nrep * a.stride;
@@ -618,10 +623,14 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         atomicAdd(&cnt_lds[b], c);
         __syncthreads();
         if (tid < 64) finish_replica(a, (r0 + tid) < a.R, r0 + tid, cnt_lds[tid]);
-        uint4 *dst = (uint4 *)((uint64_t *)a.lattice + (size_t)blockIdx.x * a.stride);
+        uint4 *lo = (uint4 *)((unsigned *)a.lattice + (size_t)(2 * blockIdx.x) * a.stride);
+        uint4 *hi = lo + a.stride / 4;
         const uint4 *src = (const uint4 *)lat_lds;
-        const int n16 = a.stride / 2;
-        for (int i = tid; i < n16; i += blockDim.x) dst[i] = src[i];
+        for (int i = tid; i < a.stride / 4; i += blockDim.x) {
+            const uint4 p = src[2 * i], q = src[2 * i + 1];
+            lo[i] = make_uint4(p.x, p.z, q.x, q.z);
+            hi[i] = make_uint4(p.y, p.w, q.y, q.w);
+        }
     } else if constexpr (LAT == LAT_MULTIHALF) {
         __syncthreads();
         // thread t counts bit (t & 31) over sites t>>5, t>>5 + S, ...
@@ -633,9 +642,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         atomicAdd(&cnt_lds[b], c);
         __syncthreads();
         if (tid < 32) finish_replica(a, (r0 + tid) < a.R, r0 + tid, cnt_lds[tid]);
-        unsigned *dst = (unsigned *)((uint64_t *)a.lattice + (size_t)(blockIdx.x >> 1) * a.stride) + (blockIdx.x & 1);
-        const unsigned *src = (const unsigned *)lat_lds;
-        for (int i = tid; i < a.stride; i += blockDim.x) dst[2 * i] = src[i];
+        uint4 *dst = (uint4 *)((unsigned *)a.lattice + (size_t)blockIdx.x * a.stride);
+        const uint4 *src = (const uint4 *)lat_lds;
+        for (int i = tid; i < a.stride / 4; i += blockDim.x) dst[i] = src[i];
     } else {
         if constexpr (LAT == LAT_BITS) {
             __syncthreads();
@@ -854,7 +863,10 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
 // (replica or group, element) so that any R fits the launch limits.
 // `per` consecutive replicas share one source grid (1: one grid per replica;
 // R / ninputs: committor inputs replicated on the device).
-__global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict__ dst, int R, int N, int stride,
+// Layout 2 in HBM: for group g (replicas 64g .. 64g+63) two dword planes of
+// `stride` dwords: plane 2g holds bit b < 32 of site s as bit b of dword s,
+// plane 2g + 1 the bits 32..63.  A split workgroup stages one plane.
+__global__ void pack_multispin(const int *__restrict__ src, unsigned *__restrict__ dst, int R, int N, int stride,
                                int per) {
     const int G = (R + 63) / 64;
     const size_t total = (size_t)G * N;
@@ -864,17 +876,19 @@ __global__ void pack_multispin(const int *__restrict__ src, uint64_t *__restrict
         const int nb = min(64, R - 64 * g);
         for (int b = 0; b < nb; ++b)
             if (src[(size_t)((64 * g + b) / per) * N + site] == 1) w |= 1ull << b;
-        dst[(size_t)g * stride + site] = w;
+        dst[(size_t)(2 * g) * stride + site] = (unsigned)w;
+        dst[(size_t)(2 * g + 1) * stride + site] = (unsigned)(w >> 32);
     }
 }
 
 template <typename T>
-__global__ void unpack_multispin(const uint64_t *__restrict__ src, T *__restrict__ dst, int R, int N, int stride) {
+__global__ void unpack_multispin(const unsigned *__restrict__ src, T *__restrict__ dst, int R, int N, int stride) {
     const int G = (R + 63) / 64;
     const size_t total = (size_t)G * N;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
         const int g = (int)(i / N), site = (int)(i - (size_t)g * N);
-        const uint64_t w = src[(size_t)g * stride + site];
+        const uint64_t w = (uint64_t)src[(size_t)(2 * g) * stride + site] |
+                           ((uint64_t)src[(size_t)(2 * g + 1) * stride + site] << 32);
         const int nb = min(64, R - 64 * g);
         for (int b = 0; b < nb; ++b) dst[(size_t)(64 * g + b) * N + site] = ((w >> b) & 1ull) ? T(1) : T(-1);
     }
@@ -1025,7 +1039,7 @@ __global__ void pack_gridstates(const void *__restrict__ lat, int method, int st
             const int r = (int)(i / N), site = (int)(i - (size_t)r * N);
             unsigned up;
             if (method == 2) {
-                up = (unsigned)((((const uint64_t *)lat)[(size_t)(r >> 6) * stride + site] >> (r & 63)) & 1ull);
+                up = (((const unsigned *)lat)[(size_t)(r >> 5) * stride + site] >> (r & 31)) & 1u;
             } else if (method == 1) {
                 up = (((const unsigned *)lat)[(size_t)r * stride + (site >> 5)] >> (site & 31)) & 1u;
             } else {
@@ -1046,7 +1060,7 @@ __global__ void gather_spins8(const void *__restrict__ lat, int method, int stri
         const int r = idx[k];
         int up;
         if (method == 2) {
-            up = (int)((((const uint64_t *)lat)[(size_t)(r >> 6) * stride + site] >> (r & 63)) & 1ull);
+            up = (int)((((const unsigned *)lat)[(size_t)(r >> 5) * stride + site] >> (r & 31)) & 1u);
         } else if (method == 1) {
             up = (int)((((const unsigned *)lat)[(size_t)r * stride + (site >> 5)] >> (site & 31)) & 1u);
         } else {
