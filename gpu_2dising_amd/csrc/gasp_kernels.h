@@ -39,13 +39,32 @@ enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3 };
 enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
 enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
-// LDS header shared by all layouts: 10 thresholds (80 B, padded to 128),
-// 64 per-replica counters and the 64 row/column neighbourhood masks of the
+// LDS header shared by all layouts: the 10 thresholds as entries 8 s01 + nup
+// of a 16-entry table (128 B; s01 = centre spin bit, nup = up neighbours), 64
+// per-replica counters and the 64 row/column neighbourhood masks of the
 // conflict filter plus a zero entry.
 constexpr int kLdsThr = 0;
 constexpr int kLdsCnt = 128;
 constexpr int kLdsNbr = 384;
 constexpr int kLdsLattice = 912;  // 384 + 65 masks of 8 bytes, rounded up to 16
+
+// LDS by absolute address.  The sweep kernels declare no static LDS, so their
+// dynamic LDS (smem) starts at LDS address 0 and smem + x is address x.  A
+// pointer formed from the integer lets the compiler fold constant offsets
+// into the ds instruction; offsets from smem itself cost an add of its base,
+// which is only resolved after instruction selection.
+typedef __attribute__((address_space(3))) unsigned lds_u32;
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+__device__ __forceinline__ unsigned lds_load32(unsigned addr) { return *(const lds_u32 *)addr; }
+__device__ __forceinline__ uint64_t lds_load64(unsigned addr) { return *(const lds_u64 *)addr; }
+__device__ __forceinline__ void lds_xor32(unsigned addr, unsigned v) {
+    __hip_atomic_fetch_xor((lds_u32 *)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// LDS address of smem (0 unless the kernel gains static LDS)
+__device__ __forceinline__ unsigned lds_origin(unsigned char *smem) {
+    return (unsigned)(uintptr_t)(lds_u8 *)smem;
+}
 
 struct SweepArgs {
     void *lattice;           // native layout (see engine)
@@ -85,12 +104,22 @@ __device__ __forceinline__ void philox4x32_10(uint32_t &c0, uint32_t &c1, uint32
     }
 }
 
+// Wave mask of the lanes where p holds.  HIP's __ballot(int) widens p to an
+// integer in a vector register and compares it again; the builtin takes the
+// compare's lane mask as it is.
+__device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Exact site maps and periodic stencils.
 //  Philox: site = floor(w*N / 2^32), computed as row = floor(w*L / 2^32) and
 //  col = floor(frac*L) from the fractional part, which is the same integer;
 //  for L = 2^lg this is w >> (32 - 2 lg).
 //  MT19937: row = floor(L * (u+0.5)*2^-32) = ((2u+1)*L) >> 33 (src/mc_cpu.c:51),
 //  which for L = 2^lg is u >> (32 - lg).
+// Stencil positions are site indices times the geometry's unit U: U = 1 for
+// site indices, U = 4 for byte offsets of a dword-per-site LDS lattice (the
+// split multispin layout), which then serve as LDS addresses directly.
+// row and col are the site's row and column mod 64, all the conflict filter
+// uses (exact for L <= 64; the filter is off for other L not a power of two).
 struct Stencil {
     int c, up, dn, lt, rt;
     int row, col;
@@ -98,43 +127,48 @@ struct Stencil {
 __device__ __forceinline__ bool in_stencil(int x, const Stencil &s) {
     return (x == s.c) | (x == s.up) | (x == s.dn) | (x == s.lt) | (x == s.rt);
 }
-// Wave mask of the lanes whose stencil holds the (wave-uniform) site x: one
-// compare-to-mask per stencil site, ORed as scalars.
+// Wave mask of the lanes whose stencil holds the (wave-uniform) position x:
+// one compare-to-mask per stencil site, ORed as scalars.
 __device__ __forceinline__ uint64_t ballot_in_stencil(int x, const Stencil &s) {
-    return __ballot(x == s.c) | __ballot(x == s.up) | __ballot(x == s.dn) | __ballot(x == s.lt) |
-           __ballot(x == s.rt);
+    return wave_mask(x == s.c) | wave_mask(x == s.up) | wave_mask(x == s.dn) | wave_mask(x == s.lt) |
+           wave_mask(x == s.rt);
 }
 
-template <bool POW2>
+template <bool POW2, int U = 1>
 struct Geometry {
+    static_assert(U == 1 || U == 4, "positions in sites or in dword byte offsets");
+    static constexpr int kLgU = U == 4 ? 2 : 0;
     int L, N, lg;  // lg = log2(L) when POW2
 
     __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {
         Stencil s;
         s.row = row;
         s.col = col;
-        s.c = row * L + col;
-        s.up = (row == 0) ? s.c + (N - L) : s.c - L;
-        s.dn = (row == L - 1) ? s.c - (N - L) : s.c + L;
-        s.lt = (col == 0) ? s.c + (L - 1) : s.c - 1;
-        s.rt = (col == L - 1) ? s.c - (L - 1) : s.c + 1;
+        s.c = (row * L + col) * U;
+        s.up = (row == 0) ? s.c + (N - L) * U : s.c - L * U;
+        s.dn = (row == L - 1) ? s.c - (N - L) * U : s.c + L * U;
+        s.lt = (col == 0) ? s.c + (L - 1) * U : s.c - U;
+        s.rt = (col == L - 1) ? s.c - (L - 1) * U : s.c + U;
         return s;
     }
-    __device__ __forceinline__ Stencil from_site_pow2(int c) const {
-        const int m = N - 1, cm = L - 1;
+    // cu = site * U for L = 2^lg: wrap by masks
+    __device__ __forceinline__ Stencil from_pos_pow2(int cu) const {
+        const int m = N * U - 1, cm = (L - 1) * U;
         Stencil s;
-        s.row = c >> lg;
-        s.col = c & cm;
-        s.c = c;
-        s.up = (c - L) & m;
-        s.dn = (c + L) & m;
-        s.lt = (c & ~cm) | ((c - 1) & cm);
-        s.rt = (c & ~cm) | ((c + 1) & cm);
+        s.row = (int)__builtin_amdgcn_ubfe((uint32_t)cu, (uint32_t)(lg + kLgU), 6u);
+        s.col = (int)__builtin_amdgcn_ubfe((uint32_t)cu, (uint32_t)kLgU, (uint32_t)(lg < 6 ? lg : 6));
+        s.c = cu;
+        s.up = (cu - L * U) & m;
+        s.dn = (cu + L * U) & m;
+        s.lt = (cu & ~cm) | ((cu - U) & cm);
+        s.rt = (cu & ~cm) | ((cu + U) & cm);
         return s;
     }
     __device__ __forceinline__ Stencil philox(uint32_t w) const {
         if constexpr (POW2) {
-            return from_site_pow2((int)(w >> (32 - 2 * lg)));
+            // site = w >> (32 - 2 lg); site * 4 keeps two more bits, cleared
+            if constexpr (U == 1) return from_pos_pow2((int)(w >> (32 - 2 * lg)));
+            else return from_pos_pow2((int)((w >> (32 - 2 * lg - kLgU)) & ~(uint32_t)(U - 1)));
         } else {
             const uint64_t t = (uint64_t)w * (uint32_t)L;
             const int row = (int)(t >> 32);
@@ -144,7 +178,7 @@ struct Geometry {
     }
     __device__ __forceinline__ Stencil mt(uint32_t urow, uint32_t ucol) const {
         if constexpr (POW2) {
-            return from_site_pow2((int)(((urow >> (32 - lg)) << lg) | (ucol >> (32 - lg))));
+            return from_pos_pow2((int)((((urow >> (32 - lg)) << lg) | (ucol >> (32 - lg))) << kLgU));
         } else {
             const int row = (int)((((uint64_t)urow << 1) + 1ull) * (uint32_t)L >> 33);
             const int col = (int)((((uint64_t)ucol << 1) + 1ull) * (uint32_t)L >> 33);
@@ -159,7 +193,9 @@ struct Geometry {
 //   dword holding it (other waves flip other bits of the same word).
 // Layout 3 (multispin, split word): the same HBM words, but a workgroup holds
 //   one 32-bit half of each (32 replicas): a dword per site in LDS, half the
-//   LDS of layout 2, so twice the workgroups for the same replicas.
+//   LDS of layout 2, so twice the workgroups for the same replicas.  Positions
+//   are the dwords' byte offsets (geometry unit 4); a lane reads the whole
+//   dword at LDS address kLdsLattice + position and takes bit rl.
 // Layout 1 (bit lattice): replica-private region of `stride` dwords in LDS,
 //   bit (site & 31) of dword (site >> 5).
 // Layout 0 (HBM): one byte (0/1) per site in global memory, replica stride
@@ -175,7 +211,7 @@ struct Lattice {
         if constexpr (LAT == LAT_MULTISPIN) {
             return (base[site * 8] >> (rl & 7)) & 1;
         } else if constexpr (LAT == LAT_MULTIHALF) {
-            return (base[site * 4] >> (rl & 7)) & 1;
+            return (int)__builtin_amdgcn_ubfe(lds_load32(kLdsLattice + (unsigned)site), (uint32_t)rl, 1u);
         } else if constexpr (LAT == LAT_BITS) {
             return (base[site >> 3] >> (site & 7)) & 1;
         } else {
@@ -187,7 +223,7 @@ struct Lattice {
             // word `site` is dwords 2*site (bits 0..31) and 2*site+1 (bits 32..63)
             atomicXor(dw + 2 * site + (rl >> 5), 1u << (rl & 31));
         } else if constexpr (LAT == LAT_MULTIHALF) {
-            atomicXor(dw + site, 1u << (rl & 31));
+            lds_xor32(kLdsLattice + (unsigned)site, 1u << rl);
         } else if constexpr (LAT == LAT_BITS) {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
         } else {
@@ -203,18 +239,24 @@ struct Attempt {
     bool acc;
 };
 
+// acc: the draw accepts the flip (whether the attempt is enabled is the caller's)
 template <int LAT>
-__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stencil &st, uint32_t ua, bool en,
-                                            const uint64_t *thr_lds) {
+__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stencil &st, uint32_t ua) {
     Attempt t;
     t.st = st;
     t.s01 = lat.spin(st.c);
-    const unsigned nup = (unsigned)(lat.spin(st.up) + lat.spin(st.dn)) + (unsigned)(lat.spin(st.lt) + lat.spin(st.rt));
-    // byte offset of entry 5 s01 + nup: 40 s01 + 8 nup
-    const unsigned off = (unsigned)t.s01 * 40u + (nup << 3);
-    const uint64_t T = *(const uint64_t *)((const char *)thr_lds + off);
-    t.acc = en && ((uint64_t)ua < T);
+    const unsigned n3 = (unsigned)(lat.spin(st.up) + lat.spin(st.dn) + lat.spin(st.lt));
+    // byte offset of table entry 8 s01 + nup: shifts and adds, no multiply or select
+    const unsigned j3 = ((unsigned)t.s01 << 3) + n3;
+    const unsigned off = (j3 + (unsigned)lat.spin(st.rt)) << 3;
+    const uint64_t T = lds_load64(kLdsThr + off);
+    t.acc = (uint64_t)ua < T;
     return t;
+}
+
+// Host threshold j = 5 s01 + nup (j < 10) goes to LDS table entry 8 s01 + nup.
+__device__ __forceinline__ void stage_thresholds(const SweepArgs &a, uint64_t *thr_lds, int tid) {
+    if (tid < 10) thr_lds[tid < 5 ? tid : tid + 3] = a.thr[tid];
 }
 
 // ------------------------------------------------------------ conflict filter
@@ -302,14 +344,34 @@ __device__ __forceinline__ void group_excl_or(uint32_t (&w)[NW], int slot) {
     }
 }
 
-__device__ __forceinline__ uint32_t fold32(uint64_t m) { return (uint32_t)m | (uint32_t)(m >> 32); }
-
 // The filter tests rows and columns only: once the exact loop visits only the
 // groups that hold a candidate, the diagonal projections (two more masks per
 // attempt and two more scanned dwords) cost more than the extra candidates
-// they remove.  With them the candidate test is exact for one earlier
-// accepted attempt; without them it also admits the four diagonal neighbours.
-constexpr bool kFilterDiagonals = false;
+// they remove.  The test admits the stencil offsets and the four diagonal
+// neighbours of each earlier accepted attempt.
+
+// Bits strictly above the lowest set bit of x inside each aligned S-bit group
+// (none for a group without a set bit): per group, ~(x ^ (x - 1)).  The even
+// and the odd groups are done as one 64-bit subtraction each; the lowest bit
+// of each group of the other parity is set as a guard, so that the borrow out
+// of an empty group stops there (out of the top group it leaves the word).
+template <int S>
+__device__ __forceinline__ uint64_t group_after(uint64_t x) {
+    if constexpr (S == 64) {
+        return ~(x ^ (x - 1ull));  // x = 0: ~(all ones) = 0
+    } else {
+        constexpr uint64_t kOnes = (S == 32) ? 0xffffffffull : ((1ull << S) - 1ull);
+        uint64_t lo = 0ull, ev = 0ull;
+#pragma unroll
+        for (int p = 0; p < 64; p += S) lo |= 1ull << p;
+#pragma unroll
+        for (int p = 0; p < 64; p += 2 * S) ev |= kOnes << p;
+        const uint64_t od = ~ev, lo_ev = lo & ev, lo_od = lo & od;
+        const uint64_t e = (x & ev) | lo_od;
+        const uint64_t o = (x & od) | lo_ev;
+        return (ev & ~(e ^ (e - lo_ev))) | (od & ~(o ^ (o - lo_od)));
+    }
+}
 
 // Lanes of the S-lane groups (aligned, S < 64) that hold at least one set bit
 // of the wave mask x: OR each group into its first lane, keep those bits,
@@ -351,10 +413,10 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
         }
     }
     if (a.check == CHECK_COUNT) {
-        const uint64_t b = __ballot(gt_up);
+        const uint64_t b = wave_mask(gt_up);
         if ((threadIdx.x & 63) == 0 && b) atomicAdd(&a.counts[0], (int)__popcll(b));
     } else if (a.check == CHECK_FATES) {
-        const uint64_t ba = __ballot(is_a), bb = __ballot(is_b);
+        const uint64_t ba = wave_mask(is_a), bb = wave_mask(is_b);
         if ((threadIdx.x & 63) == 0) {
             if (ba) atomicAdd(&a.counts[1], (int)__popcll(ba));
             if (bb) atomicAdd(&a.counts[2], (int)__popcll(bb));
@@ -395,11 +457,15 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     // (replicas past R, and when freezing those with a decided fate, do not)
     const bool live = r < a.R;
     const bool active = live && !(a.freeze && a.fate[r] != -1);
+    // absolute LDS addresses assume smem at address 0: otherwise leave the
+    // lattice alone and report NaN magnetisations
+    if (lds_origin(smem) != 0u) {
+        if (live && slot == 0) a.mag[r] = __builtin_nanf("");
+        return;
+    }
 
     // ---------------------------------------------------------- prologue
-#pragma unroll
-    for (int i = 0; i < 10; ++i)
-        if (tid == i) thr_lds[i] = a.thr[i];
+    stage_thresholds(a, thr_lds, tid);
     if (tid < 64) cnt_lds[tid] = 0u;
     // entry 64 is zero: the mask of an attempt that was not accepted
     for (int t = tid; t <= 64; t += blockDim.x) nbr_lds[t] = t < 64 ? nbr_mask(t, a.L) : 0ull;
@@ -450,13 +516,14 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     const uint32_t e = (uint32_t)a.n_attempts + b;
     uint32_t l0 = active ? b : e;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    const Geometry<POW2> geo{L, N, a.lg};
+    constexpr int U = (LAT == LAT_MULTIHALF) ? 4 : 1;  // stencil positions: byte offsets in layout 3
+    const Geometry<POW2, U> geo{L, N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS)
         drw = a.draws + (size_t)(live ? r : 0) * (3 * a.n_attempts);
 
     // ---------------------------------------------------------- rounds
-    while (__ballot(l0 < e) != 0ull) {
+    while (wave_mask(l0 < e) != 0ull) {
         const uint32_t q = (l0 >> 1) + (uint32_t)slot;
         const uint32_t lA = 2 * q, lB = lA + 1;
         const bool enA = (lA >= l0) && (lA < e);
@@ -488,68 +555,54 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             uaB = v2;
         }
 
-        const Attempt A = evaluate<LAT>(lat, sA, uaA, enA, thr_lds);
-        const Attempt B = evaluate<LAT>(lat, sB, uaB, enB, thr_lds);
+        const Attempt A = evaluate<LAT>(lat, sA, uaA);
+        const Attempt B = evaluate<LAT>(lat, sB, uaB);
 
-        // B conflicts with the same lane's accepted A
-        const bool ownB = A.acc && in_stencil(A.st.c, B.st);
-        bool validA, validB;
+        // Accepted attempts as wave masks, used from here on as lane
+        // predicates through inverse ballots (kept in scalar registers, where
+        // per-lane booleans across the branches below cost selects).
+        // (each ballot of a single compare: a ballot of a combined boolean
+        // costs a select and a compare to clear the inactive lanes)
+        const uint64_t eA = wave_mask(lA >= l0) & wave_mask(lA < e);
+        const uint64_t eB = wave_mask(lB >= l0) & wave_mask(lB < e);
+        uint64_t mA = eA & wave_mask(A.acc), mB = eB & wave_mask(B.acc);
+        // lanes whose B holds the same lane's accepted A in its stencil
+        uint64_t oB = mA & ballot_in_stencil(A.st.c, B.st);
+        // opaque to the optimiser, which would otherwise fold the inverse
+        // ballots below back into per-lane booleans
+        asm volatile("" : "+s"(mA), "+s"(mB), "+s"(oB));
         if constexpr (S > 1) {
             // Wave masks: cA / cB = lanes whose attempt A / B has an earlier
             // accepted attempt of another lane of its group in its stencil.
             // The loop runs over the accepted attempts only (wave-uniform);
             // `later` (the lanes after i in i's group) is a scalar mask.
-            const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
-            uint64_t cA = 0ull, cB = __ballot(ownB);
+            uint64_t cA = 0ull, cB = oB;
             uint64_t m = mA | mB;
             // The filter pays up to 32 lanes per group (a 32-slot split launch
             // at L=64, 16384 replicas: 42.5 -> 37.9 ms); over a whole wave
             // (method 0 at 64 slots) it cost more than it saved (39.1 -> 40.6).
             if constexpr (S <= 32) {
-                // Filter (see nbr_mask): union the projection masks of the
-                // accepted attempts of the earlier lanes of the group; an
-                // attempt within 1 on every projection tested (rows and
-                // columns as 64-bit masks; with kFilterDiagonals also the two
-                // diagonals folded to 32 bits) is a candidate.  Without a
-                // candidate in the wave the exact loop is skipped.
+                // Filter (see nbr_mask): union the row and column masks of
+                // the accepted attempts of the earlier lanes of the group; an
+                // attempt within 1 of that union on both is a candidate.
+                // Without a candidate in the wave the exact loop is skipped.
                 if (use_filter && m != 0ull) {
-                    constexpr int NP = kFilterDiagonals ? 4 : 2;
-                    int pA[4], pB[4];  // row, col, (row + col) mod L, (row - col) mod L
-                    pA[0] = A.st.row, pA[1] = A.st.col, pB[0] = B.st.row, pB[1] = B.st.col;
-                    if constexpr (kFilterDiagonals) {
-                        if constexpr (POW2) {
-                            pA[2] = (pA[0] + pA[1]) & (L - 1), pA[3] = (pA[0] - pA[1]) & (L - 1);
-                            pB[2] = (pB[0] + pB[1]) & (L - 1), pB[3] = (pB[0] - pB[1]) & (L - 1);
-                        } else {
-                            pA[2] = pA[0] + pA[1], pA[3] = pA[0] - pA[1];
-                            pB[2] = pB[0] + pB[1], pB[3] = pB[0] - pB[1];
-                            pA[2] -= pA[2] >= L ? L : 0, pA[3] += pA[3] < 0 ? L : 0;
-                            pB[2] -= pB[2] >= L ? L : 0, pB[3] += pB[3] < 0 ? L : 0;
-                        }
-                    }
-                    uint64_t mAk[4], mBk[4];
-#pragma unroll
-                    for (int k = 0; k < NP; ++k) {
-                        // not accepted: the zero entry (a select, not a masked read)
-                        mAk[k] = nbr_lds[A.acc ? (pA[k] & 63) : 64];
-                        mBk[k] = nbr_lds[B.acc ? (pB[k] & 63) : 64];
-                    }
-                    const uint64_t rm = mAk[0] | mBk[0], km = mAk[1] | mBk[1];
-                    uint32_t w[NP == 4 ? 6 : 4];
-                    w[0] = (uint32_t)rm, w[1] = (uint32_t)(rm >> 32), w[2] = (uint32_t)km, w[3] = (uint32_t)(km >> 32);
-                    if constexpr (kFilterDiagonals) w[4] = fold32(mAk[2] | mBk[2]), w[5] = fold32(mAk[3] | mBk[3]);
+                    // not accepted: the zero entry (a select, not a masked read)
+                    const bool accA = __builtin_amdgcn_inverse_ballot_w64(mA);
+                    const bool accB = __builtin_amdgcn_inverse_ballot_w64(mB);
+                    const uint64_t rm = lds_load64(kLdsNbr + 8u * (unsigned)(accA ? A.st.row : 64)) |
+                                        lds_load64(kLdsNbr + 8u * (unsigned)(accB ? B.st.row : 64));
+                    const uint64_t km = lds_load64(kLdsNbr + 8u * (unsigned)(accA ? A.st.col : 64)) |
+                                        lds_load64(kLdsNbr + 8u * (unsigned)(accB ? B.st.col : 64));
+                    uint32_t w[4] = {(uint32_t)rm, (uint32_t)(rm >> 32), (uint32_t)km, (uint32_t)(km >> 32)};
                     group_excl_or<S>(w, slot);
                     const uint64_t er = ((uint64_t)w[1] << 32) | w[0], ek = ((uint64_t)w[3] << 32) | w[2];
-                    bool sA = ((er >> (pA[0] & 63)) & (ek >> (pA[1] & 63)) & 1ull) != 0ull;
-                    bool sB = ((er >> (pB[0] & 63)) & (ek >> (pB[1] & 63)) & 1ull) != 0ull;
-                    if constexpr (kFilterDiagonals) {
-                        sA = sA && ((w[4] >> (pA[2] & 31)) & (w[5] >> (pA[3] & 31)) & 1u);
-                        sB = sB && ((w[4] >> (pB[2] & 31)) & (w[5] >> (pB[3] & 31)) & 1u);
-                    }
+                    const bool sA = ((er >> A.st.row) & (ek >> A.st.col) & 1ull) != 0ull;
+                    const bool sB = ((er >> B.st.row) & (ek >> B.st.col) & 1ull) != 0ull;
                     // the exact loop then visits only the accepted attempts
                     // of groups that hold a candidate (a group without one
                     // has no conflict)
-                    const uint64_t cand = __ballot(sA || sB);
+                    const uint64_t cand = wave_mask(sA || sB);
                     m = cand ? (m & group_span<S>(cand)) : 0ull;
                 }
             }
@@ -569,40 +622,39 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                     cB |= ballot_in_stencil(x, B.st) & later;
                 }
             }
-            const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << gbase) - 1ull);
-            validA = (((cA | cB) & below) == 0ull) && !((cA >> lane) & 1ull);
-            validB = validA && !((cB >> lane) & 1ull);
-        } else {
-            validA = true;
-            validB = !ownB;
-        }
-
-        if (A.acc && validA) lat.flip(A.st.c, A.s01);
-        if (B.acc && validB) lat.flip(B.st.c, B.s01);
-
-        const bool invA = enA && !validA, invB = enB && !validB;
-        if constexpr (S > 1) {
-            const uint64_t iA = __ballot(invA), iB = __ballot(invB);
-            const uint64_t gm = (S == 64) ? ~0ull : (((1ull << S) - 1ull) << gbase);
-            const uint64_t x = (iA | iB) & gm;
-            const uint32_t q0 = l0 >> 1;
-            if (x == 0ull) {
-                const uint32_t nxt = 2 * (q0 + S);
-                l0 = nxt < e ? nxt : e;
+            // Invalid attempts, as wave masks: every lane after the first
+            // conflicting lane of its group (both attempts), and at a lane
+            // its own conflicts (an invalid A also invalidates its B).
+            const uint64_t invA = group_after<S>(cA | cB) | cA;
+            const uint64_t invB = invA | cB;
+            if (__builtin_amdgcn_inverse_ballot_w64(mA & ~invA)) lat.flip(A.st.c, A.s01);
+            if (__builtin_amdgcn_inverse_ballot_w64(mB & ~invB)) lat.flip(B.st.c, B.s01);
+            // The valid attempts of a group are a prefix of its order A, B,
+            // A, B, ... (an attempt is invalid once an earlier one conflicts);
+            // the next round starts after that prefix.  Disabled attempts
+            // before l0 are valid (nothing precedes them) and those at or past
+            // e end past e, so the clamp to e covers them.
+            uint32_t P;
+            if constexpr (S == 64) {
+                P = (uint32_t)(__popcll(~invA) + __popcll(~invB));
             } else {
-                const int f = __builtin_ctzll(x);
-                l0 = 2 * (q0 + (uint32_t)(f - gbase)) + (((iA >> f) & 1ull) ? 0u : 1u);
+                constexpr uint32_t kOnes = (S == 32) ? 0xffffffffu : ((1u << S) - 1u);
+                P = (uint32_t)__popc((uint32_t)(~invA >> gbase) & kOnes) +
+                    (uint32_t)__popc((uint32_t)(~invB >> gbase) & kOnes);
             }
+            const uint32_t nxt = (l0 & ~1u) + P;
+            l0 = nxt < e ? nxt : e;
             if constexpr (LAT == LAT_HBM) {
                 // lanes of a replica must see each other's byte stores next round
-                if (__ballot((A.acc && validA) || (B.acc && validB)) != 0ull) {
+                if (((mA & ~invA) | (mB & ~invB)) != 0ull) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
             }
         } else {
-            (void)invA;
-            if (invB) {
+            if (__builtin_amdgcn_inverse_ballot_w64(mA)) lat.flip(A.st.c, A.s01);
+            if (__builtin_amdgcn_inverse_ballot_w64(mB & ~oB)) lat.flip(B.st.c, B.s01);
+            if (__builtin_amdgcn_inverse_ballot_w64(eB & oB)) {
                 l0 = lB;
             } else {
                 const uint32_t nxt = lA + 2;
@@ -709,10 +761,12 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     const int r = blockIdx.x;  // grid = R workgroups
     // uniform over the workgroup
     const bool active = !(a.freeze && a.fate[r] != -1);
+    if (lds_origin(smem) != 0u) {  // as in sweep_kernel
+        if (tid == 0) a.mag[r] = __builtin_nanf("");
+        return;
+    }
 
-#pragma unroll
-    for (int i = 0; i < 10; ++i)
-        if (tid == i) thr_lds[i] = a.thr[i];
+    stage_thresholds(a, thr_lds, tid);
     if (tid == 0) cnt_lds[0] = 0u;
     __syncthreads();
 
@@ -764,12 +818,14 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
             sB = geo.mt(v0, v1);
             uaB = v2;
         }
-        const Attempt A = evaluate<LAT_HBM>(lat, sA, uaA, enA, thr_lds);
-        const Attempt B = evaluate<LAT_HBM>(lat, sB, uaB, enB, thr_lds);
+        Attempt A = evaluate<LAT_HBM>(lat, sA, uaA);
+        Attempt B = evaluate<LAT_HBM>(lat, sB, uaB);
+        A.acc = A.acc && enA;
+        B.acc = B.acc && enB;
 
         // conflicts inside the wave (the whole wave is one group)
-        const uint64_t mA = __ballot(A.acc), mB = __ballot(B.acc);
-        uint64_t cA = 0ull, cB = __ballot(A.acc && in_stencil(A.st.c, B.st));
+        const uint64_t mA = wave_mask(A.acc), mB = wave_mask(B.acc);
+        uint64_t cA = 0ull, cB = wave_mask(A.acc && in_stencil(A.st.c, B.st));
         uint64_t m = mA | mB;
         while (m) {
             const int i = __builtin_ctzll(m);
@@ -814,7 +870,7 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
         // this wave's first invalid attempt (local index 2 lane + (B ? 1 : 0); 128 = none)
         const bool validA = (((cA | cB) & lt_mask) == 0ull) && !((cA >> lane) & 1ull);
         const bool validB = validA && !((cB >> lane) & 1ull);
-        const uint64_t iA = __ballot(enA && !validA), iB = __ballot(enB && !validB);
+        const uint64_t iA = wave_mask(enA && !validA), iB = wave_mask(enB && !validB);
         const uint64_t x = iA | iB;
         if (lane == 0) {
             int f = 128;
@@ -1098,7 +1154,7 @@ __global__ void check_cv_kernel(const float *cv, int R, int check, double dn_thr
             is_b = f == 1;
         }
     }
-    const uint64_t b0 = __ballot(gt_up), ba = __ballot(is_a), bb = __ballot(is_b);
+    const uint64_t b0 = wave_mask(gt_up), ba = wave_mask(is_a), bb = wave_mask(is_b);
     if ((threadIdx.x & 63) == 0) {
         if (check == CHECK_COUNT && b0) atomicAdd(&counts[0], (int)__popcll(b0));
         if (check == CHECK_FATES) {
