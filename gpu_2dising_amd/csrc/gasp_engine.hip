@@ -232,15 +232,32 @@ static int layout_stride(int method, int N) {
     return ((N + 15) / 16) * 16;                  // bytes per replica
 }
 
+// Entries of sweep_hbm_waves' LDS conflict table (a power of two): one per
+// site, up to what the LDS of a CU holds for the workgroups it should run at
+// once (R workgroups over the CUs, at most four per CU), and at most 32768
+// (128 KB).  Fewer entries than sites only adds conflicts (sites share
+// entries); at the batched L = 256 shape (one workgroup per CU) two sites
+// share each entry.
+static int hbm_table_entries(int N, int R) {
+    DeviceInfo d = device_info();
+    const int ncu = d.ok ? d.ncu : 256;
+    const size_t lds = d.ok ? d.lds_block : 65536;
+    const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
+    const size_t budget = (lds / (size_t)per_cu - (size_t)kLdsLattice) / 4;
+    int M = 1;
+    while (M < N && M < 32768 && (size_t)M * 2 <= budget) M *= 2;
+    return M;
+}
+
 // Method 2: `waves` is the word split (1: 64 replicas per workgroup, 2: 32).
-static size_t lds_bytes_for(int method, int N, int S, int waves) {
+static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
     const int stride = layout_stride(method, N);
     if (method == 2) return (size_t)kLdsLattice + (waves == 2 ? 4 : 8) * (size_t)stride;
     if (method == 1) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
     }
-    if (S > 64) return (size_t)kLdsWaveList + (size_t)waves * (128 * 4 + 8);  // sweep_hbm_waves lists
+    if (S > 64) return (size_t)kLdsLattice + 4 * (size_t)hbm_table_entries(N, R);  // sweep_hbm_waves table
     return (size_t)kLdsLattice;
 }
 
@@ -306,8 +323,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     if (method == 0 && S > 64) W = S / 64;  // one replica per workgroup
     if (method == 1) {
         // enough slots so that the replica regions of one workgroup fit
-        while (S < 64 && lds_bytes_for(1, N, S, W) > lds_block) S <<= 1;
-        while (W > 1 && lds_bytes_for(1, N, S, W) > lds_block) W >>= 1;
+        while (S < 64 && lds_bytes_for(1, N, S, W, R) > lds_block) S <<= 1;
+        while (W > 1 && lds_bytes_for(1, N, S, W, R) > lds_block) W >>= 1;
     }
     *S_out = S;
     *W_out = W;
@@ -378,6 +395,7 @@ static int ilog2(int L) {
 struct gasp_engine {
     int L = 0, N = 0, R = 0, method = 0;
     int S = 1, W = 1, stride = 0, rpw = 0, nblocks = 0, block = 0;
+    int tbl = 1;  // sweep_hbm_waves: conflict-table entries
     size_t lds = 0;
     void *d_lat = nullptr;
     size_t lat_bytes = 0;
@@ -471,7 +489,8 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->stride = layout_stride(method, e->N);
     DeviceInfo di = device_info();
     if (di.ok) e->ncu = di.ncu;
-    e->lds = lds_bytes_for(method, e->N, e->S, e->W);
+    e->lds = lds_bytes_for(method, e->N, e->S, e->W, R);
+    e->tbl = (method == 0 && e->S > 64) ? hbm_table_entries(e->N, R) : 1;
     if (method == 2) {
         const int groups = (R + 63) / 64;
         e->rpw = e->W == 2 ? 32 : 64;
@@ -605,6 +624,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.counts = e->d_counts;
     a.check = check;
     a.freeze = e->freeze;
+    a.tbl_mask = (unsigned)(e->tbl - 1);
     a.dn_thr = dn;
     a.up_thr = up;
     SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L));
@@ -1243,7 +1263,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
     int bs = 1, bw = 1;
     DeviceInfo di = device_info();
     for (auto [S, W] : cand) {
-        if (di.ok && lds_bytes_for(method, L * L, S, W) > di.lds_block) continue;
+        if (di.ok && lds_bytes_for(method, L * L, S, W, ngrids) > di.lds_block) continue;
         g_tune_slots = S;
         g_tune_waves = W;
         gasp_engine *e = nullptr;
@@ -1363,7 +1383,7 @@ extern "C" int select_gpu_method(int L, int ngrids, int threadsPerBlock, int gpu
     size_t lds = 65536;
     if (hipGetDeviceProperties(&prop, gpu_device < 0 ? 0 : gpu_device) == hipSuccess) lds = prop.sharedMemPerBlock;
     const int N = L * L;
-    if (lds_bytes_for(2, N, 1, 1) <= lds) return 2;
+    if (lds_bytes_for(2, N, 1, 1, 64) <= lds) return 2;
     if ((size_t)kLdsLattice + (size_t)layout_stride(1, N) * 4 <= lds) return 1;
     return 0;
 }

@@ -83,6 +83,7 @@ struct SweepArgs {
     int *counts;             // out: [0] #(m>up), [1] #fate 0, [2] #fate 1
     int check;
     int freeze;              // skip replicas whose fate is already decided
+    unsigned tbl_mask;       // sweep_hbm_waves: entries of its LDS conflict table - 1 (a power of two - 1)
     double dn_thr, up_thr;
 };
 
@@ -239,16 +240,26 @@ struct Attempt {
     bool acc;
 };
 
-// acc: the draw accepts the flip (whether the attempt is enabled is the caller's)
+// The five spins of a stencil, read before any is used: a kernel reads both
+// attempts' spins first, so that on the HBM layout (atomic loads, which the
+// compiler keeps in order with the threshold read) the ten reads share one
+// round trip to L2 instead of two.
+struct Spins {
+    int c, up, dn, lt, rt;
+};
 template <int LAT>
-__device__ __forceinline__ Attempt evaluate(const Lattice<LAT> &lat, const Stencil &st, uint32_t ua) {
+__device__ __forceinline__ Spins read_spins(const Lattice<LAT> &lat, const Stencil &st) {
+    return Spins{lat.spin(st.c), lat.spin(st.up), lat.spin(st.dn), lat.spin(st.lt), lat.spin(st.rt)};
+}
+// acc: the draw accepts the flip (whether the attempt is enabled is the caller's)
+__device__ __forceinline__ Attempt decide(const Stencil &st, const Spins &x, uint32_t ua) {
     Attempt t;
     t.st = st;
-    t.s01 = lat.spin(st.c);
-    const unsigned n3 = (unsigned)(lat.spin(st.up) + lat.spin(st.dn) + lat.spin(st.lt));
+    t.s01 = x.c;
+    const unsigned n3 = (unsigned)(x.up + x.dn + x.lt);
     // byte offset of table entry 8 s01 + nup: shifts and adds, no multiply or select
-    const unsigned j3 = ((unsigned)t.s01 << 3) + n3;
-    const unsigned off = (j3 + (unsigned)lat.spin(st.rt)) << 3;
+    const unsigned j3 = ((unsigned)x.c << 3) + n3;
+    const unsigned off = (j3 + (unsigned)x.rt) << 3;
     const uint64_t T = lds_load64(kLdsThr + off);
     t.acc = (uint64_t)ua < T;
     return t;
@@ -555,8 +566,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             uaB = v2;
         }
 
-        const Attempt A = evaluate<LAT>(lat, sA, uaA);
-        const Attempt B = evaluate<LAT>(lat, sB, uaB);
+        const Spins xA = read_spins<LAT>(lat, sA), xB = read_spins<LAT>(lat, sB);
+        const Attempt A = decide(sA, xA, uaA);
+        const Attempt B = decide(sB, xB, uaB);
 
         // Accepted attempts as wave masks, used from here on as lane
         // predicates through inverse ballots (kept in scalar registers, where
@@ -730,30 +742,40 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 }
 
 // ------------------------------------------------------------ multi-wave HBM sweep
-// Layout 0 with S = 64 W lanes per replica spread over the W waves (2 or 4) of
-// one workgroup, one replica per workgroup.  For large lattices with few
+// Layout 0 with S = 64 W lanes per replica spread over the W waves (2, 4 or 8)
+// of one workgroup, one replica per workgroup.  For large lattices with few
 // replicas (the L = 256 group of a batched run has 256), one wave per replica
 // leaves three of each CU's four SIMDs idle and the round latency exposed.
 // Slot t = 64 k + lane of wave k evaluates the attempt pair (2t, 2t+1) of the
 // round, exactly as sweep_kernel does with S = 64 W, so the trajectory is the
-// same sequential one.  Conflicts inside a wave are found as in sweep_kernel;
-// across waves, an attempt conflicts with every accepted attempt of an earlier
-// wave whose site its stencil holds.  A round has three barriers: after the
-// waves publish their accepted sites in LDS, after each publishes its first
-// invalid attempt (the round commits everything before the earliest), and
-// after the flips, so that the next round's reads see them.
-constexpr int kLdsWaveList = kLdsLattice;  // [W][128] accepted sites, [W] counts, [W] first invalid
-
+// same sequential one.
+//
+// Conflicts go through an LDS table instead of lane-to-lane compares: each
+// accepted attempt writes its order in the round (key 2t or 2t + 1) into the
+// entry of its site with an atomic minimum, and after a barrier every attempt
+// reads the entries of its five stencil sites; one below its own key means an
+// earlier accepted attempt may have changed what it read.  The table has a
+// power-of-two number of entries, and site x uses entry x & tbl_mask: when
+// the lattice has more sites than entries, sites that share an entry can only
+// add conflicts, and a spurious conflict only sends an attempt (and the rest
+// of the round) to the next round, so the trajectory is unchanged.  The round
+// commits every attempt before the first conflicting one.  The serial
+// compare chains this replaces (one readlane and ten compares per accepted
+// attempt, in the wave and then across the earlier waves' lists) were two
+// thirds of a round at the batched L = 256 shape.
+// A round has three barriers: after the table entries are written, after each
+// wave publishes its first conflicting attempt, and after the flips and the
+// table's reset, so that the next round reads both.
 template <int RNG, int W, bool POW2>
 __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     static_assert(W == 2 || W == 4 || W == 8, "two, four or eight waves per replica");
     constexpr int S = 64 * W;
+    constexpr unsigned kNone = 0xffffffffu;
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
     unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);
-    int *acc_lds = (int *)(smem + kLdsWaveList);
-    int *nacc_lds = acc_lds + W * 128;
-    int *finv_lds = nacc_lds + W;
+    int *finv_lds = (int *)(smem + kLdsNbr);   // [W] first conflicting attempt of each wave
+    constexpr unsigned kTbl = kLdsLattice;      // LDS address of the conflict table
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -768,6 +790,8 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
 
     stage_thresholds(a, thr_lds, tid);
     if (tid == 0) cnt_lds[0] = 0u;
+    const unsigned hm = a.tbl_mask;
+    for (unsigned i = (unsigned)tid; i <= hm; i += S) *(lds_u32 *)(kTbl + 4u * i) = kNone;
     __syncthreads();
 
     Lattice<LAT_HBM> lat;
@@ -784,7 +808,7 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     const Geometry<POW2> geo{a.L, a.N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS) drw = a.draws + (size_t)r * (3 * a.n_attempts);
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const unsigned keyA = 2u * (unsigned)tid, keyB = keyA + 1u;
 
     while (l0 < e) {
         const uint32_t q0 = l0 >> 1;
@@ -818,59 +842,31 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
             sB = geo.mt(v0, v1);
             uaB = v2;
         }
-        Attempt A = evaluate<LAT_HBM>(lat, sA, uaA);
-        Attempt B = evaluate<LAT_HBM>(lat, sB, uaB);
-        A.acc = A.acc && enA;
-        B.acc = B.acc && enB;
+        const Spins xA = read_spins<LAT_HBM>(lat, sA), xB = read_spins<LAT_HBM>(lat, sB);
+        const Attempt A = decide(sA, xA, uaA), B = decide(sB, xB, uaB);
+        const bool accA = A.acc && enA, accB = B.acc && enB;
 
-        // conflicts inside the wave (the whole wave is one group)
-        const uint64_t mA = wave_mask(A.acc), mB = wave_mask(B.acc);
-        uint64_t cA = 0ull, cB = wave_mask(A.acc && in_stencil(A.st.c, B.st));
-        uint64_t m = mA | mB;
-        while (m) {
-            const int i = __builtin_ctzll(m);
-            m &= m - 1;
-            const uint64_t later = ~((2ull << i) - 1ull);  // i = 63: later = 0
-            if ((mA >> i) & 1ull) {
-                const int x = __builtin_amdgcn_readlane(A.st.c, i);
-                cA |= ballot_in_stencil(x, A.st) & later;
-                cB |= ballot_in_stencil(x, B.st) & later;
-            }
-            if ((mB >> i) & 1ull) {
-                const int x = __builtin_amdgcn_readlane(B.st.c, i);
-                cA |= ballot_in_stencil(x, A.st) & later;
-                cB |= ballot_in_stencil(x, B.st) & later;
-            }
-        }
-        // publish this wave's accepted sites (at most 128)
-        const int pos = (int)(__popcll(mA & lt_mask) + __popcll(mB & lt_mask));
-        if (A.acc) acc_lds[wave * 128 + pos] = A.st.c;
-        if (B.acc) acc_lds[wave * 128 + pos + (A.acc ? 1 : 0)] = B.st.c;
-        if (lane == 0) nacc_lds[wave] = (int)(__popcll(mA) + __popcll(mB));
+        // mark: the lowest key of the round's accepted attempts per entry
+        if (accA) __hip_atomic_fetch_min((lds_u32 *)(kTbl + 4u * ((unsigned)A.st.c & hm)), keyA,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (accB) __hip_atomic_fetch_min((lds_u32 *)(kTbl + 4u * ((unsigned)B.st.c & hm)), keyB,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __syncthreads();
-        // every attempt of this wave comes after every attempt of earlier
-        // waves; their lists are read into registers once (lane i holds
-        // entries i and 64 + i), then walked with readlane
-        for (int k = 0; k < wave; ++k) {
-            const int n = __builtin_amdgcn_readfirstlane(nacc_lds[k]);
-            const int v0 = acc_lds[k * 128 + lane];
-            const int v1 = acc_lds[k * 128 + 64 + lane];
-            const int n0 = n < 64 ? n : 64;
-            for (int i = 0; i < n0; ++i) {
-                const int x = __builtin_amdgcn_readlane(v0, i);
-                cA |= ballot_in_stencil(x, A.st);
-                cB |= ballot_in_stencil(x, B.st);
-            }
-            for (int i = 64; i < n; ++i) {
-                const int x = __builtin_amdgcn_readlane(v1, i - 64);
-                cA |= ballot_in_stencil(x, A.st);
-                cB |= ballot_in_stencil(x, B.st);
-            }
-        }
-        // this wave's first invalid attempt (local index 2 lane + (B ? 1 : 0); 128 = none)
-        const bool validA = (((cA | cB) & lt_mask) == 0ull) && !((cA >> lane) & 1ull);
-        const bool validB = validA && !((cB >> lane) & 1ull);
-        const uint64_t iA = wave_mask(enA && !validA), iB = wave_mask(enB && !validB);
+        // test: an entry of the stencil below the attempt's own key (the
+        // attempt's own mark is its key, not below it)
+        const unsigned mAx = min(min(min(lds_load32(kTbl + 4u * ((unsigned)A.st.c & hm)),
+                                         lds_load32(kTbl + 4u * ((unsigned)A.st.up & hm))),
+                                     min(lds_load32(kTbl + 4u * ((unsigned)A.st.dn & hm)),
+                                         lds_load32(kTbl + 4u * ((unsigned)A.st.lt & hm)))),
+                                 lds_load32(kTbl + 4u * ((unsigned)A.st.rt & hm)));
+        const unsigned mBx = min(min(min(lds_load32(kTbl + 4u * ((unsigned)B.st.c & hm)),
+                                         lds_load32(kTbl + 4u * ((unsigned)B.st.up & hm))),
+                                     min(lds_load32(kTbl + 4u * ((unsigned)B.st.dn & hm)),
+                                         lds_load32(kTbl + 4u * ((unsigned)B.st.lt & hm)))),
+                                 lds_load32(kTbl + 4u * ((unsigned)B.st.rt & hm)));
+        // this wave's first conflicting enabled attempt (local index 2 lane + (B ? 1 : 0); 128 = none)
+        const uint64_t iA = wave_mask(mAx < keyA) & wave_mask(enA);
+        const uint64_t iB = wave_mask(mBx < keyB) & wave_mask(enB);
         const uint64_t x = iA | iB;
         if (lane == 0) {
             int f = 128;
@@ -881,23 +877,26 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
             finv_lds[wave] = f;
         }
         __syncthreads();
-        // the round's first invalid attempt, as an offset from attempt 2 q0
+        // the round's first conflicting attempt, as an offset from attempt 2 q0
         int F = 2 * S;
         for (int k = W - 1; k >= 0; --k) {
             const int f = __builtin_amdgcn_readfirstlane(finv_lds[k]);
             if (f < 128) F = 128 * k + f;
         }
         const int tA = 128 * wave + 2 * lane;
-        if (A.acc && tA < F) lat.flip(A.st.c, A.s01);
-        if (B.acc && tA + 1 < F) lat.flip(B.st.c, B.s01);
+        if (accA && tA < F) lat.flip(A.st.c, A.s01);
+        if (accB && tA + 1 < F) lat.flip(B.st.c, B.s01);
+        // every reader has read: clear what this round marked
+        if (accA) *(lds_u32 *)(kTbl + 4u * ((unsigned)A.st.c & hm)) = kNone;
+        if (accB) *(lds_u32 *)(kTbl + 4u * ((unsigned)B.st.c & hm)) = kNone;
         if (F < 2 * S) {
             l0 = 2 * q0 + (uint32_t)F;
         } else {
             const uint32_t nxt = 2 * (q0 + S);
             l0 = nxt < e ? nxt : e;
         }
-        // flips visible to every wave before the next round's reads; the
-        // lists are rewritten only after this barrier
+        // flips visible to every wave, and the table clear, before the next
+        // round's reads and marks
         __syncthreads();
     }
 
