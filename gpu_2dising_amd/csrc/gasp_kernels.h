@@ -120,7 +120,7 @@ __device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_
 // site indices, U = 4 for byte offsets of a dword-per-site LDS lattice (the
 // split multispin layout), which then serve as LDS addresses directly.
 // row and col are the site's row and column mod 64, all the conflict filter
-// uses (exact for L <= 64; the filter is off for other L not a power of two).
+// uses.
 struct Stencil {
     int c, up, dn, lt, rt;
     int row, col;
@@ -143,8 +143,8 @@ struct Geometry {
 
     __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {
         Stencil s;
-        s.row = row;
-        s.col = col;
+        s.row = row & 63;
+        s.col = col & 63;
         s.c = (row * L + col) * U;
         s.up = (row == 0) ? s.c + (N - L) * U : s.c - L * U;
         s.dn = (row == L - 1) ? s.c - (N - L) * U : s.c + L * U;
@@ -272,28 +272,21 @@ __device__ __forceinline__ void stage_thresholds(const SweepArgs &a, uint64_t *t
 
 // ------------------------------------------------------------ conflict filter
 // An attempt at (r', c') conflicts with an accepted attempt at (r, c) iff
-// (dr, dc) = (r' - r, c' - c) mod L is one of the five stencil offsets.  Then
-// each of the projections r, c, r + c and r - c (mod L) differs by at most 1;
-// conversely, if all four differ by at most 1, (dr, dc) is a stencil offset
-// (the diagonal offsets fail r + c or r - c).  The filter tests the four
-// projections of each attempt against the unions, over the earlier accepted
-// attempts of its group, of their "within 1" sets: exact for one earlier
-// accepted attempt, conservative for several.
-// nbr_mask(t): bits of the values within periodic distance 1 of t, folded
-// mod 64.  Exact for L <= 64; for power-of-two L > 64, entry (t & 63) holds
-// the fold of the values around t (64 divides L, so (t +- 1 mod L) & 63 =
-// ((t & 63) +- 1) & 63).
-__device__ __forceinline__ uint64_t nbr_mask(int t, int L) {
-    int rm1, rp1;
-    if (L <= 64) {
-        if (t >= L) return 0ull;
-        rm1 = (t == 0) ? L - 1 : t - 1;
-        rp1 = (t == L - 1) ? 0 : t + 1;
-    } else {
-        rm1 = (t - 1) & 63;
-        rp1 = (t + 1) & 63;
+// (dr, dc) = (r' - r, c' - c) mod L is one of the five stencil offsets, so
+// only if its row and its column each differ by at most 1 (periodically).
+// The filter tests rows and columns as 64-bit masks over the values mod 64.
+// nbr_mask(k): the union, over every value t < L with t mod 64 = k, of the
+// values within periodic distance 1 of t, folded mod 64.  Exact for L <= 64;
+// for larger L the fold only adds bits (for powers of two each t gives the
+// same bits), so the test stays conservative: a true conflict always passes.
+__device__ __forceinline__ uint64_t nbr_mask(int k, int L) {
+    uint64_t m = 0ull;
+    for (int t = k; t < L; t += 64) {
+        const int rm1 = (t == 0) ? L - 1 : t - 1;
+        const int rp1 = (t == L - 1) ? 0 : t + 1;
+        m |= (1ull << (rm1 & 63)) | (1ull << (t & 63)) | (1ull << (rp1 & 63));
     }
-    return (1ull << rm1) | (1ull << t) | (1ull << rp1);
+    return m;
 }
 
 // DPP row_shr:K (lane l reads lane l - K of its 16-lane row; 0 outside it).
@@ -480,8 +473,6 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     if (tid < 64) cnt_lds[tid] = 0u;
     // entry 64 is zero: the mask of an attempt that was not accepted
     for (int t = tid; t <= 64; t += blockDim.x) nbr_lds[t] = t < 64 ? nbr_mask(t, a.L) : 0ull;
-    // the row / column filter needs exact (L <= 64) or aligned (64 | L) folding
-    const bool use_filter = POW2 || a.L <= 64;
     if constexpr (LAT == LAT_MULTISPIN) {
         // HBM: the group's two dword planes (bits 0..31, then 32..63 of each
         // site's word), a.stride dwords each (N rounded up to 4); LDS: one
@@ -598,7 +589,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                 // the accepted attempts of the earlier lanes of the group; an
                 // attempt within 1 of that union on both is a candidate.
                 // Without a candidate in the wave the exact loop is skipped.
-                if (use_filter && m != 0ull) {
+                if (m != 0ull) {
                     // not accepted: the zero entry (a select, not a masked read)
                     const bool accA = __builtin_amdgcn_inverse_ballot_w64(mA);
                     const bool accB = __builtin_amdgcn_inverse_ballot_w64(mB);

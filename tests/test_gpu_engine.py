@@ -253,6 +253,23 @@ def test_method2_largest_lattices(gpu, engine_lib, oracle, L, S, W):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("method,S,W,L", [(2, 16, 2, 96), (2, 32, 2, 100), (1, 16, 1, 96), (2, 8, 2, 200)])
+@pytest.mark.parametrize("start", ["random", "down"])
+def test_folded_filter_large_L(gpu, engine_lib, oracle, method, S, W, L, start):
+    """L > 64 not a power of two: the conflict filter folds rows and columns
+    mod 64 (an entry is the union over every row with that residue), which
+    may only add candidates.  Random starts at beta = 0.3 (many conflicts)
+    and all-down starts at beta = 0.54 (the filter skips most rounds)."""
+    G, O = engine_lib, oracle
+    R, seed = 70, 6061
+    beta = 0.3 if start == "random" else BETA
+    print(f"[fold] method={method} S={S} W={W} L={L} start={start}", flush=True)
+    g0 = _start(start, R, L, seed)
+    got, mag = _run_engine(G, L, R, method, S, g0, seed, False, beta, H, [1, 2], W)
+    ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
+    assert np.array_equal(got, ref)
+
+
 def test_method2_whole_words_refused_past_L142(gpu, engine_lib):
     """At L = 143 one group's 64-bit words exceed the 160 KB of LDS per
     workgroup: the engine reports it instead of launching."""
