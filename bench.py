@@ -60,7 +60,8 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
     independent Philox streams, run until all fates are decided against
     dn = -0.90 / up = +0.90 (the CLI thresholds) or max_sweeps; replicas with a
     decided fate stop sweeping.  pB per input and its bootstrap error
-    (bootstrap_errbar, libc rand(), as the reference) are computed on the host."""
+    (bootstrap_errbar's estimator on glibc's rand() sequence, as the
+    reference, on host threads) are computed on the host."""
     import ctypes as C
 
     inputs = committor_inputs(n_inputs, L)
@@ -87,11 +88,21 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
     nB = (f == 1).sum(1)
     nres = (f >= 0).sum(1)
     pB = np.where(nres > 0, nB / np.maximum(nres, 1), np.nan).astype(np.float32)
+    # bootstrap_errbar's estimator on glibc's rand() sequence from srand(1)
+    # (glibc's unseeded state), bit for bit as successive calls would draw it;
+    # the library jumps its own copy of the sequence to each input, so the
+    # inputs run on 16 host threads
     t1 = time.perf_counter()
-    lib = G.lib()
-    errs = [lib.bootstrap_errbar(per, per, np.ascontiguousarray(f[i]).ctypes.data_as(C.POINTER(C.c_int)))
-            for i in range(n_inputs)]
+    errs = G.bootstrap_libc(f, per, seed=1, nthreads=16)
     t_boot = time.perf_counter() - t1
+    # spot check against the real libc rand(): the first two inputs, called in order after srand(1)
+    lib = G.lib()
+    libc = C.CDLL("libc.so.6")
+    libc.srand.argtypes = [C.c_uint]
+    libc.srand(1)
+    seq = [lib.bootstrap_errbar(per, per, np.ascontiguousarray(f[i]).ctypes.data_as(C.POINTER(C.c_int)))
+           for i in range(2)]
+    boot_checked = bool(np.array_equal(np.array(seq), errs[:2], equal_nan=True))
     t1 = time.perf_counter()
     errs_gpu = G.bootstrap_gpu(f, per, seed=1)
     t_boot_gpu = time.perf_counter() - t1
@@ -103,11 +114,13 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
         "inputs": n_inputs, "replicas_per_input": per, "sweeps": sweeps,
         "unresolved": int(R_c - c[1] - c[2]), "gpu_seconds": t_gpu, "kernel_seconds": kern_ms / 1e3,
         "bootstrap_seconds": t_boot,
+        "bootstrap_threads": 16,
+        "bootstrap_matches_sequential_libc": boot_checked,
         "value_gpu_bootstrap": n_inputs / (t_gpu + t_boot_gpu),
         "bootstrap_gpu_seconds": t_boot_gpu,
         "mean_bootstrap_error_gpu": float(np.nanmean(errs_gpu)),
         "pB_quantiles": [float(x) for x in np.nanquantile(pB, [0.0, 0.25, 0.5, 0.75, 1.0])],
-        "mean_bootstrap_error": float(np.mean(errs)),
+        "mean_bootstrap_error": float(np.nanmean(errs)),
         "workload": "512 grids: -1 sea, +1 disc of radius U[4,14] at a random centre, 5% of spins "
                     "flipped; 1024 replicas each; thresholds -0.90/+0.90; <= 20000 sweeps",
         "_pB": pB,
@@ -160,7 +173,7 @@ def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
     rs = np.random.default_rng(seed)
     tuned = []
     for (Lg, Rg, m) in BATCH_GROUPS:
-        s, w, rate = G.autotune(m, Lg, Rg, 2)
+        s, w, rate = G.autotune(m, Lg, Rg, 4)
         tuned.append({"L": Lg, "replicas": Rg, "method": m, "slots": s, "waves": w, "probe_rate": rate})
     engines, rngs, checks, dn, up = [], [], [], [], []
     for k, (Lg, Rg, m) in enumerate(BATCH_GROUPS):

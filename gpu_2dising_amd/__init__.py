@@ -35,7 +35,7 @@ EXPORTS = (
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
     "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
-    "gasp_bootstrap_errbar_gpu",
+    "gasp_bootstrap_errbar_gpu", "gasp_bootstrap_errbar_libc",
     "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
     "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
     "gasp_cnn_set_data", "gasp_cnn_train_epoch",
@@ -167,6 +167,8 @@ def lib() -> C.CDLL:
     L.gasp_autotune.restype = C.c_int
     L.gasp_bootstrap_errbar_gpu.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, vp]
     L.gasp_bootstrap_errbar_gpu.restype = C.c_int
+    L.gasp_bootstrap_errbar_libc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int, vp]
+    L.gasp_bootstrap_errbar_libc.restype = C.c_int
     i8p = C.POINTER(C.c_int8)
     for name in ("gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
                  "gasp_cnn_get_params", "gasp_cnn_predict", "gasp_cnn_gradient", "gasp_cnn_train_step",
@@ -423,6 +425,19 @@ def bootstrap_gpu(fates: np.ndarray, m: int, seed: int = 1) -> np.ndarray:
         raise ValueError("fates must be [ninputs, n]")
     out = np.empty(f.shape[0], dtype=np.float64)
     check(lib().gasp_bootstrap_errbar_gpu(f.ctypes.data, 0, f.shape[0], f.shape[1], m, seed, out.ctypes.data))
+    return out
+
+
+def bootstrap_libc(fates: np.ndarray, m: int, seed: int = 1, nthreads: int = 0) -> np.ndarray:
+    """bootstrap_errbar (src/bootstrap.c:6-63) for fates [ninputs, n], bit for
+    bit as successive calls after srand(seed) with glibc's rand(), on host
+    threads (the library jumps its own copy of the rand() sequence)."""
+    f = np.ascontiguousarray(fates, dtype=np.int32)
+    if f.ndim != 2:
+        raise ValueError("fates must be [ninputs, n]")
+    out = np.empty(f.shape[0], dtype=np.float64)
+    check(lib().gasp_bootstrap_errbar_libc(f.ctypes.data, f.shape[0], f.shape[1], m, seed, nthreads,
+                                           out.ctypes.data))
     return out
 
 

@@ -1228,10 +1228,10 @@ extern "C" double gasp_graph_ms(gasp_graph *g) { return g ? g->ms : 0.0; }
 
 // ============================================================ autotuner
 // Times each candidate launch shape on a throw-away engine (all spins -1,
-// Philox, probe_sweeps sweeps after one warm-up sweep) and keeps the fastest
-// for (method, L, ngrids).  Candidates: slots per replica S (lanes that
-// speculate on one replica) and, for methods 0/1, waves per workgroup W; for
-// method 0 also 128 and 256 slots (a replica over 2 or 4 waves).
+// Philox, probe_sweeps sweeps after as many warm-up sweeps) and keeps the
+// fastest for (method, L, ngrids).  Candidates: slots per replica S (lanes
+// that speculate on one replica) and, for methods 0/1, waves per workgroup W;
+// for method 0 also 128, 256 and 512 slots (a replica over 2, 4 or 8 waves).
 extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
                              double *best_rate) {
     clear_error();
@@ -1276,8 +1276,10 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
             continue;
         }
         double rate = -1.0;
-        if (!engine_set_grids_impl(e, grids.data(), 0, ngrids) && !engine_advance(e, (uint64_t)e->N, 0, 0, 0) &&
-            !engine_sync(e)) {
+        // warm-up as long as the timed part: from the all-down start the
+        // first sweeps accept least, which flatters wide speculation groups
+        if (!engine_set_grids_impl(e, grids.data(), 0, ngrids) &&
+            !engine_advance(e, (uint64_t)probe_sweeps * e->N, 0, 0, 0) && !engine_sync(e)) {
             const double ms0 = e->kernel_ms;
             if (!engine_advance(e, (uint64_t)probe_sweeps * e->N, 0, 0, 0) && !engine_sync(e)) {
                 const double ms = e->kernel_ms - ms0;

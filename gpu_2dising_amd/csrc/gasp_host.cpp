@@ -6,6 +6,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <array>
+#include <thread>
 #include <vector>
 
 #include "gasp_internal.h"
@@ -160,7 +163,138 @@ double bootstrap_errbar(int ngrids, int m, int *grid_fate_host) {
     return sqrt(num / m);
 }
 
+// bootstrap_errbar for `ninputs` successive inputs (n fates each), exactly as
+// `ninputs` calls of bootstrap_errbar in a row after srand(seed), with the
+// inputs spread over threads; see libc_window below.
+int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, int m, unsigned seed, int nthreads,
+                               double *out);
+
 }  // extern "C"
+
+// ------------------------------------------------------------ libc rand() stream
+// glibc's rand() is random_r's TYPE_3 additive feedback generator: srand(s)
+// sets r[0] = s (1 if s = 0, as a 32-bit signed value) and
+// r[i] = 16807 r[i-1] mod (2^31 - 1) for i = 1..30 (Schrage's method),
+// r[i] = r[i-31] for i = 31..33, and r[i] = r[i-3] + r[i-31] mod 2^32 for
+// i >= 34; call k (k = 0, 1, ...) returns r[k + 344] >> 1.  The recurrence is
+// linear, with characteristic polynomial x^31 - x^28 - 1: if
+// x^d = sum_j a_j x^j modulo it, then r[t + d] = sum_j a_j r[t + j] for t >= 3,
+// so the stream jumps ahead by any number of calls.
+namespace {
+using Poly = std::array<uint32_t, 31>;
+
+Poly poly_mulmod(const Poly &a, const Poly &b) {
+    uint32_t prod[61] = {};
+    for (int i = 0; i < 31; ++i)
+        for (int j = 0; j < 31; ++j) prod[i + j] += a[i] * b[j];
+    for (int k = 60; k >= 31; --k) {  // x^k = x^(k-3) + x^(k-31)
+        prod[k - 3] += prod[k];
+        prod[k - 31] += prod[k];
+    }
+    Poly r;
+    std::copy(prod, prod + 31, r.begin());
+    return r;
+}
+
+Poly x_pow(uint64_t d) {
+    Poly r{}, b{};
+    r[0] = 1u;
+    b[1] = 1u;
+    for (; d; d >>= 1) {
+        if (d & 1) r = poly_mulmod(r, b);
+        b = poly_mulmod(b, b);
+    }
+    return r;
+}
+
+// r[s .. s+30] for s = 313 + call, the 31 values before the one call `call` uses
+void libc_window(unsigned seed, uint64_t call, uint32_t w[31]) {
+    int32_t r0[34];
+    int32_t word = seed == 0 ? 1 : (int32_t)seed;
+    r0[0] = word;
+    for (int i = 1; i < 31; ++i) {
+        const int32_t hi = word / 127773, lo = word % 127773;
+        word = 16807 * lo - 2836 * hi;
+        if (word < 0) word += 2147483647;
+        r0[i] = word;
+    }
+    for (int i = 31; i < 34; ++i) r0[i] = r0[i - 31];
+    uint32_t base[31];  // r[3 .. 33]
+    for (int j = 0; j < 31; ++j) base[j] = (uint32_t)r0[3 + j];
+    Poly a = x_pow(310 + call);  // r[313 + call] from the window at t = 3
+    for (int k = 0; k < 31; ++k) {
+        uint32_t v = 0u;
+        for (int j = 0; j < 31; ++j) v += a[j] * base[j];
+        w[k] = v;
+        // a <- a x, reduced: the x^31 term becomes x^28 + 1
+        const uint32_t top = a[30];
+        for (int j = 30; j > 0; --j) a[j] = a[j - 1];
+        a[0] = top;
+        a[28] += top;
+    }
+}
+
+// The reference's estimator for one input, drawing from the stream in w
+// (advanced in place): m resamples of n indices rand() % n.
+double bootstrap_one(const int *f, int n, int m, uint32_t w[31], std::vector<uint32_t> &buf,
+                     std::vector<double> &store) {
+    const size_t need = (size_t)n;
+    const uint64_t M = UINT64_MAX / (uint64_t)n + 1u;
+    buf.resize(need + 31);
+    for (int im = 0; im < m; ++im) {
+        std::copy(w, w + 31, buf.begin());
+        for (size_t i = 31; i < need + 31; ++i) buf[i] = buf[i - 3] + buf[i - 31];
+        std::copy(buf.end() - 31, buf.end(), w);
+        // (r >> 1) % n by Lemire's exact 32-bit remainder: M = ceil(2^64 / n),
+        // then the high word of (M (r >> 1) mod 2^64) n; counts without branches
+        int nA = 0, nB = 0;
+        for (size_t i = 0; i < need; ++i) {
+            const uint64_t low = M * (uint64_t)(buf[31 + i] >> 1);
+            const int g = (int)(((unsigned __int128)low * (uint64_t)n) >> 64);
+            const int x = f[g];
+            nA += (x == 0);
+            nB += (x == 1);
+        }
+        store[im] = (double)nB / (double)(nA + nB);
+    }
+    double mean = 0.0, num = 0.0;
+    for (int im = 0; im < m; ++im) mean = mean + store[im];
+    mean = mean / m;
+    for (int im = 0; im < m; ++im) num = num + (store[im] - mean) * (store[im] - mean);
+    return sqrt(num / m);
+}
+}  // namespace
+
+extern "C" int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, int m, unsigned seed,
+                                          int nthreads, double *out) {
+    gasp::clear_error();
+    if (!fates || !out || ninputs < 0 || n < 1 || m < 1)
+        return gasp::set_error(GASP_ERR_ARG, "gasp_bootstrap_errbar_libc: bad argument");
+    if (nthreads < 1) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    nthreads = std::max(1, std::min(nthreads, ninputs));
+    const uint64_t per_input = (uint64_t)n * (uint64_t)m;  // rand() calls of one input
+    auto work = [&](int i0, int i1) {
+        uint32_t w[31];
+        libc_window(seed, (uint64_t)i0 * per_input, w);
+        std::vector<uint32_t> buf;
+        std::vector<double> store((size_t)m);
+        for (int i = i0; i < i1; ++i) out[i] = bootstrap_one(fates + (size_t)i * n, n, m, w, buf, store);
+    };
+    std::vector<std::thread> pool;
+    const int chunk = (ninputs + nthreads - 1) / nthreads;
+    try {
+        for (int t = 1; t < nthreads; ++t) {
+            const int i0 = t * chunk, i1 = std::min(ninputs, i0 + chunk);
+            if (i0 < i1) pool.emplace_back(work, i0, i1);
+        }
+    } catch (...) {
+        for (auto &th : pool) th.join();
+        return gasp::set_error(GASP_ERR_NOMEM, "gasp_bootstrap_errbar_libc: cannot start threads");
+    }
+    work(0, std::min(ninputs, chunk));
+    for (auto &th : pool) th.join();
+    return 0;
+}
 
 int gasp::append_gridstates(int L, int ngrids, int isweep, const uint8_t *bits, const char *filename) {
     const char *name = filename ? filename : "gridstates.bin";

@@ -56,10 +56,11 @@ constexpr int kLdsLattice = 912;  // 384 + 65 masks of 8 bytes, rounded up to 16
 typedef __attribute__((address_space(3))) unsigned lds_u32;
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
-__device__ __forceinline__ unsigned lds_load32(unsigned addr) { return *(const lds_u32 *)addr; }
-__device__ __forceinline__ uint64_t lds_load64(unsigned addr) { return *(const lds_u64 *)addr; }
+__device__ __forceinline__ lds_u32 *lds_ptr32(unsigned addr) { return (lds_u32 *)(uintptr_t)addr; }
+__device__ __forceinline__ unsigned lds_load32(unsigned addr) { return *lds_ptr32(addr); }
+__device__ __forceinline__ uint64_t lds_load64(unsigned addr) { return *(const lds_u64 *)(uintptr_t)addr; }
 __device__ __forceinline__ void lds_xor32(unsigned addr, unsigned v) {
-    __hip_atomic_fetch_xor((lds_u32 *)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_xor(lds_ptr32(addr), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 // LDS address of smem (0 unless the kernel gains static LDS)
 __device__ __forceinline__ unsigned lds_origin(unsigned char *smem) {
@@ -782,7 +783,7 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     stage_thresholds(a, thr_lds, tid);
     if (tid == 0) cnt_lds[0] = 0u;
     const unsigned hm = a.tbl_mask;
-    for (unsigned i = (unsigned)tid; i <= hm; i += S) *(lds_u32 *)(kTbl + 4u * i) = kNone;
+    for (unsigned i = (unsigned)tid; i <= hm; i += S) *lds_ptr32(kTbl + 4u * i) = kNone;
     __syncthreads();
 
     Lattice<LAT_HBM> lat;
@@ -838,9 +839,9 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
         const bool accA = A.acc && enA, accB = B.acc && enB;
 
         // mark: the lowest key of the round's accepted attempts per entry
-        if (accA) __hip_atomic_fetch_min((lds_u32 *)(kTbl + 4u * ((unsigned)A.st.c & hm)), keyA,
+        if (accA) __hip_atomic_fetch_min(lds_ptr32(kTbl + 4u * ((unsigned)A.st.c & hm)), keyA,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (accB) __hip_atomic_fetch_min((lds_u32 *)(kTbl + 4u * ((unsigned)B.st.c & hm)), keyB,
+        if (accB) __hip_atomic_fetch_min(lds_ptr32(kTbl + 4u * ((unsigned)B.st.c & hm)), keyB,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __syncthreads();
         // test: an entry of the stencil below the attempt's own key (the
@@ -878,8 +879,8 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
         if (accA && tA < F) lat.flip(A.st.c, A.s01);
         if (accB && tA + 1 < F) lat.flip(B.st.c, B.s01);
         // every reader has read: clear what this round marked
-        if (accA) *(lds_u32 *)(kTbl + 4u * ((unsigned)A.st.c & hm)) = kNone;
-        if (accB) *(lds_u32 *)(kTbl + 4u * ((unsigned)B.st.c & hm)) = kNone;
+        if (accA) *lds_ptr32(kTbl + 4u * ((unsigned)A.st.c & hm)) = kNone;
+        if (accB) *lds_ptr32(kTbl + 4u * ((unsigned)B.st.c & hm)) = kNone;
         if (F < 2 * S) {
             l0 = 2 * q0 + (uint32_t)F;
         } else {

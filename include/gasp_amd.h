@@ -268,6 +268,15 @@ int gasp_graph_destroy(gasp_graph *g);
  * bootstrap_errbar keeps the reference's sequence).  host_out[ninputs]. */
 int gasp_bootstrap_errbar_gpu(const int *fates, int on_device, int ninputs, int n, int m,
                               unsigned long long seed, double *host_out);
+/* bootstrap_errbar (src/bootstrap.c:6-63) for `ninputs` inputs of n fates each
+ * (host int32 [ninputs][n]), bit for bit as `ninputs` successive calls of
+ * bootstrap_errbar(n, m, ...) after srand(seed) with glibc's rand() (seed 1 is
+ * glibc's unseeded state): the library generates glibc's rand() sequence
+ * itself and jumps it ahead to each input's first draw, so the inputs run on
+ * `nthreads` host threads (0: one per core).  Does not touch the process's
+ * own rand() state.  out[ninputs]. */
+int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, int m, unsigned seed, int nthreads,
+                               double *out);
 int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
                   double *best_rate);
 

@@ -115,6 +115,30 @@ def test_bootstrap_matches_reference(engine_lib, oracle):
     assert np.array_equal(np.array(got).view(np.uint64), gold["boot_vals"].view(np.uint64))
 
 
+@pytest.mark.parametrize("seed", [1, 7, 12345, 2**31 + 5])
+def test_bootstrap_libc_batch_is_sequential_libc(engine_lib, seed):
+    """gasp_bootstrap_errbar_libc (its own copy of glibc's rand() sequence,
+    jumped ahead to each input, inputs on threads) equals successive
+    bootstrap_errbar calls on the real libc rand() after srand(seed), bit for
+    bit: inputs of 0/1/-1 fates (one all undecided: 0/0), 1 to 7 threads, so
+    that threads start at offsets of several hundred thousand calls."""
+    G = engine_lib
+    lib = G.lib()
+    libc = C.CDLL("libc.so.6")
+    libc.srand.argtypes = [C.c_uint]
+    rs = np.random.default_rng(seed % 1000)
+    ninputs, n, m = 9, 333, 211
+    f = rs.integers(-1, 2, size=(ninputs, n)).astype(np.int32)
+    f[4] = -1
+    libc.srand(seed)
+    ref = np.array([lib.bootstrap_errbar(n, m, np.ascontiguousarray(f[i]).ctypes.data_as(C.POINTER(C.c_int)))
+                    for i in range(ninputs)])
+    for nthreads in (1, 2, 3, 7):
+        got = G.bootstrap_libc(f, m, seed=seed, nthreads=nthreads)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.array_equal(got[~np.isnan(ref)], ref[~np.isnan(ref)]), nthreads
+
+
 def test_threshold_table_layout():
     """Kernel table order j = 5*s01 + (up neighbours) is the reference index
     5(s+1) + nsum + 4 = 10 s01 + 2c (s = 2 s01 - 1, nsum = 2c - 4)."""
