@@ -234,10 +234,12 @@ static int layout_stride(int method, int N) {
 
 // Entries of sweep_hbm_waves' LDS conflict table (a power of two): one per
 // site, up to what the LDS of a CU holds for the workgroups it should run at
-// once (R workgroups over the CUs, at most four per CU), and at most 32768
-// (128 KB).  Fewer entries than sites only adds conflicts (sites share
-// entries); at the batched L = 256 shape (one workgroup per CU) two sites
-// share each entry.
+// once (R workgroups over the CUs, at most four per CU), and at most 16384
+// (64 KB), which leaves room on the CU for other kernels' workgroups (in a
+// batched graph, 128 KB tables kept the LDS groups off every CU the HBM group
+// held: 11.96 ms per replay against 10.38 ms).  Fewer entries than sites only
+// adds conflicts (sites share entries); at the batched L = 256 shape four
+// sites share each entry.
 static int hbm_table_entries(int N, int R) {
     DeviceInfo d = device_info();
     const int ncu = d.ok ? d.ncu : 256;
@@ -245,7 +247,7 @@ static int hbm_table_entries(int N, int R) {
     const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
     const size_t budget = (lds / (size_t)per_cu - (size_t)kLdsLattice) / 4;
     int M = 1;
-    while (M < N && M < 32768 && (size_t)M * 2 <= budget) M *= 2;
+    while (M < N && M < 16384 && (size_t)M * 2 <= budget) M *= 2;
     return M;
 }
 
