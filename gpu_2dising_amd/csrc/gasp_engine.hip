@@ -251,10 +251,11 @@ static int hbm_table_entries(int N, int R) {
     return M;
 }
 
-// Method 2: `waves` is the word split (1: 64 replicas per workgroup, 2: 32).
+// Method 2: `waves` is the word split (1, 2, 4, 8: 64, 32, 16, 8 replicas per
+// workgroup, 8, 4, 2, 1 bytes per site in LDS).
 static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
     const int stride = layout_stride(method, N);
-    if (method == 2) return (size_t)kLdsLattice + (waves == 2 ? 4 : 8) * (size_t)stride;
+    if (method == 2) return (size_t)kLdsLattice + (size_t)(8 / waves) * (size_t)stride;
     if (method == 1) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
@@ -289,10 +290,24 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     // replicas and 16 slots a launch takes 21.5 ms split against 21.8 ms
     // whole, with 2^19 replicas and 8 slots 4.44e11 attempts/s against 4.29e11,
     // and split words fit lattices up to L=201 instead of 142.
-    if (method == 2) W = (g_tune_waves == 1 && g_tune_slots <= 16) ? 1 : 2;
-    else W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
+    // With few replicas the words split further (16 or 8 replicas per
+    // workgroup), until there is at least one workgroup per CU: 2048 replicas
+    // at 32 per workgroup would run on 64 of the 256 CUs.
+    if (method == 2) {
+        if (g_tune_waves == 1 && g_tune_slots <= 16) {
+            W = 1;
+        } else if (g_tune_waves == 2 || g_tune_waves == 4 || g_tune_waves == 8) {
+            W = g_tune_waves;
+        } else {
+            const long long groups = (R + 63) / 64;
+            W = 2;
+            while (W < 8 && groups * W < ncu) W *= 2;
+        }
+    } else {
+        W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
+    }
     // method 0 may spread a replica over 2 or 4 waves (sweep_hbm_waves)
-    int Smax = (method == 2) ? (W == 2 ? 32 : 16) : (method == 0 ? 512 : 64);
+    int Smax = (method == 2) ? (W >= 2 ? 32 : 16) : (method == 0 ? 512 : 64);
     int S;
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), Smax);
@@ -360,7 +375,7 @@ static SweepFn pick_sweep(int S) {
     if constexpr (LAT != LAT_MULTISPIN) {
         if (S == 32) return sweep_kernel<LAT, RNG, 32, P2>;
     }
-    if constexpr (LAT != LAT_MULTISPIN && LAT != LAT_MULTIHALF) {
+    if constexpr (LAT != LAT_MULTISPIN && LAT != LAT_MULTIHALF && LAT != LAT_MULTI16 && LAT != LAT_MULTI8) {
         if (S == 64) return sweep_kernel<LAT, RNG, 64, P2>;
     }
     if constexpr (LAT == LAT_HBM) {
@@ -379,6 +394,10 @@ static SweepFn pick_sweep2(int S, bool pow2) {
 static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2) {
     if (method == 2 && W == 2)
         return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2);
+    if (method == 2 && W == 4)
+        return rng == 0 ? pick_sweep2<LAT_MULTI16, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI16, RNG_DRAWS>(S, pow2);
+    if (method == 2 && W == 8)
+        return rng == 0 ? pick_sweep2<LAT_MULTI8, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI8, RNG_DRAWS>(S, pow2);
     if (method == 2)
         return rng == 0 ? pick_sweep2<LAT_MULTISPIN, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTISPIN, RNG_DRAWS>(S, pow2);
     if (method == 1)
@@ -495,7 +514,7 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->tbl = (method == 0 && e->S > 64) ? hbm_table_entries(e->N, R) : 1;
     if (method == 2) {
         const int groups = (R + 63) / 64;
-        e->rpw = e->W == 2 ? 32 : 64;
+        e->rpw = 64 / e->W;
         e->block = e->rpw * e->S;
         e->nblocks = groups * (64 / e->rpw);
         e->lat_bytes = (size_t)groups * e->stride * 8;
@@ -1241,10 +1260,18 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
         return set_error(GASP_ERR_ARG, "gasp_autotune: bad argument");
     const int save_s = g_tune_slots, save_w = g_tune_waves;
     std::vector<std::pair<int, int>> cand;
+    const int ncu_t = device_info().ok ? device_info().ncu : 256;
+    // quarter and eighth words only for few replicas (fewer than two
+    // half-word workgroups per CU)
+    const bool few = (long long)((ngrids + 63) / 64) * 2 < 2LL * ncu_t;
     for (int S = 1; S <= 64; S <<= 1) {
         if (method == 2) {
             if (S <= 16) cand.push_back({S, 1});
             if (S >= 2 && S <= 32) cand.push_back({S, 2});
+            if (few && S >= 8 && S <= 32) {
+                cand.push_back({S, 4});
+                cand.push_back({S, 8});
+            }
         } else {
             for (int W = 1; W <= 4; W <<= 1) cand.push_back({S, W});
         }

@@ -35,7 +35,7 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 constexpr uint32_t kPhiloxDomain = 0x47415350u;  // "GASP"
 
-enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3 };
+enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3, LAT_MULTI16 = 4, LAT_MULTI8 = 5 };
 enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
 enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
@@ -56,6 +56,7 @@ constexpr int kLdsLattice = 912;  // 384 + 65 masks of 8 bytes, rounded up to 16
 typedef __attribute__((address_space(3))) unsigned lds_u32;
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef __attribute__((address_space(3))) unsigned char lds_u8;
+typedef __attribute__((address_space(3))) unsigned short lds_u16;
 __device__ __forceinline__ lds_u32 *lds_ptr32(unsigned addr) { return (lds_u32 *)(uintptr_t)addr; }
 __device__ __forceinline__ unsigned lds_load32(unsigned addr) { return *lds_ptr32(addr); }
 __device__ __forceinline__ uint64_t lds_load64(unsigned addr) { return *(const lds_u64 *)(uintptr_t)addr; }
@@ -138,8 +139,8 @@ __device__ __forceinline__ uint64_t ballot_in_stencil(int x, const Stencil &s) {
 
 template <bool POW2, int U = 1>
 struct Geometry {
-    static_assert(U == 1 || U == 4, "positions in sites or in dword byte offsets");
-    static constexpr int kLgU = U == 4 ? 2 : 0;
+    static_assert(U == 1 || U == 2 || U == 4, "positions in sites or in byte offsets of 2- or 4-byte entries");
+    static constexpr int kLgU = U == 4 ? 2 : (U == 2 ? 1 : 0);
     int L, N, lg;  // lg = log2(L) when POW2
 
     __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {
@@ -198,6 +199,10 @@ struct Geometry {
 //   LDS of layout 2, so twice the workgroups for the same replicas.  Positions
 //   are the dwords' byte offsets (geometry unit 4); a lane reads the whole
 //   dword at LDS address kLdsLattice + position and takes bit rl.
+// Layouts 4 and 5 (multispin, quarter and eighth word): 16 or 8 replicas per
+//   workgroup, a 16- or 8-bit entry per site in LDS (positions are byte
+//   offsets: units 2 and 1), for few replicas spread over more CUs.  A flip
+//   XORs bit 8 (position mod 4) + rl of the dword that holds the entry.
 // Layout 1 (bit lattice): replica-private region of `stride` dwords in LDS,
 //   bit (site & 31) of dword (site >> 5).
 // Layout 0 (HBM): one byte (0/1) per site in global memory, replica stride
@@ -214,6 +219,12 @@ struct Lattice {
             return (base[site * 8] >> (rl & 7)) & 1;
         } else if constexpr (LAT == LAT_MULTIHALF) {
             return (int)__builtin_amdgcn_ubfe(lds_load32(kLdsLattice + (unsigned)site), (uint32_t)rl, 1u);
+        } else if constexpr (LAT == LAT_MULTI16) {
+            return (int)__builtin_amdgcn_ubfe(*(const lds_u16 *)(uintptr_t)(kLdsLattice + (unsigned)site),
+                                              (uint32_t)rl, 1u);
+        } else if constexpr (LAT == LAT_MULTI8) {
+            return (int)__builtin_amdgcn_ubfe(*(const lds_u8 *)(uintptr_t)(kLdsLattice + (unsigned)site),
+                                              (uint32_t)rl, 1u);
         } else if constexpr (LAT == LAT_BITS) {
             return (base[site >> 3] >> (site & 7)) & 1;
         } else {
@@ -226,6 +237,8 @@ struct Lattice {
             atomicXor(dw + 2 * site + (rl >> 5), 1u << (rl & 31));
         } else if constexpr (LAT == LAT_MULTIHALF) {
             lds_xor32(kLdsLattice + (unsigned)site, 1u << rl);
+        } else if constexpr (LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
+            lds_xor32(kLdsLattice + ((unsigned)site & ~3u), 1u << ((((unsigned)site & 3u) << 3) + (unsigned)rl));
         } else if constexpr (LAT == LAT_BITS) {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
         } else {
@@ -432,9 +445,10 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
 template <int LAT, int RNG, int S, bool POW2>
 __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
-    constexpr bool MULTI = (LAT == LAT_MULTISPIN) || (LAT == LAT_MULTIHALF);
-    // replicas per workgroup in the multispin layouts; their lattice word half
-    constexpr int RPG = (LAT == LAT_MULTIHALF) ? 32 : 64;
+    constexpr bool SPLIT = (LAT == LAT_MULTIHALF) || (LAT == LAT_MULTI16) || (LAT == LAT_MULTI8);
+    constexpr bool MULTI = (LAT == LAT_MULTISPIN) || SPLIT;
+    // replicas per workgroup in the multispin layouts (whole, half, quarter, eighth words)
+    constexpr int RPG = (LAT == LAT_MULTIHALF) ? 32 : (LAT == LAT_MULTI16) ? 16 : (LAT == LAT_MULTI8) ? 8 : 64;
     static_assert(!MULTI || RPG * S <= 1024, "workgroup too large");
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
@@ -452,7 +466,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     int rl, r0;
     if constexpr (MULTI) {
         rl = wave * (64 / S) + (lane / S);
-        r0 = (LAT == LAT_MULTIHALF) ? (int)(blockIdx.x >> 1) * 64 + 32 * (int)(blockIdx.x & 1) : blockIdx.x * 64;
+        r0 = (int)blockIdx.x * RPG;  // split: the groups' 64 replicas over 64 / RPG workgroups
     } else {
         rl = tid / S;
         r0 = blockIdx.x * a.rpw;
@@ -491,6 +505,16 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         const uint4 *src = (const uint4 *)((const unsigned *)a.lattice + (size_t)blockIdx.x * a.stride);
         uint4 *dst = (uint4 *)lat_lds;
         for (int i = tid; i < a.stride / 4; i += blockDim.x) dst[i] = src[i];
+    } else if constexpr (LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
+        // bits o .. o + RPG - 1 of plane (64 g + o) / 32 of each site's dword,
+        // where replica r0 = blockIdx.x RPG is bit o of that plane
+        const unsigned *src = (const unsigned *)a.lattice + (size_t)(r0 >> 5) * a.stride;
+        const int o = r0 & 31;
+        for (int i = tid; i < N; i += blockDim.x) {
+            const unsigned v = src[i] >> o;
+            if constexpr (LAT == LAT_MULTI16) ((unsigned short *)lat_lds)[i] = (unsigned short)v;
+            else lat_lds[i] = (unsigned char)v;
+        }
     } else if constexpr (LAT == LAT_BITS) {
         const int nrep = min(a.rpw, a.R - r0);
         const int ndw = nrep * a.stride;
@@ -519,7 +543,8 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     const uint32_t e = (uint32_t)a.n_attempts + b;
     uint32_t l0 = active ? b : e;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    constexpr int U = (LAT == LAT_MULTIHALF) ? 4 : 1;  // stencil positions: byte offsets in layout 3
+    // stencil positions: byte offsets in the split layouts (entries of 4, 2, 1 bytes)
+    constexpr int U = (LAT == LAT_MULTIHALF) ? 4 : (LAT == LAT_MULTI16) ? 2 : 1;
     const Geometry<POW2, U> geo{L, N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS)
@@ -701,6 +726,27 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         uint4 *dst = (uint4 *)((unsigned *)a.lattice + (size_t)blockIdx.x * a.stride);
         const uint4 *src = (const uint4 *)lat_lds;
         for (int i = tid; i < a.stride / 4; i += blockDim.x) dst[i] = src[i];
+    } else if constexpr (LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
+        __syncthreads();
+        // thread t counts bit (t mod RPG) over sites t / RPG, t / RPG + S, ...
+        const int b = tid & (RPG - 1);
+        const int q = tid / RPG;
+        unsigned c = 0;
+        for (int site = q; site < N; site += S) {
+            const unsigned v = (LAT == LAT_MULTI16) ? (unsigned)((const unsigned short *)lat_lds)[site]
+                                                    : (unsigned)lat_lds[site];
+            c += (v >> b) & 1u;
+        }
+        atomicAdd(&cnt_lds[b], c);
+        __syncthreads();
+        if (tid < RPG) finish_replica(a, (r0 + tid) < a.R, r0 + tid, cnt_lds[tid]);
+        // the workgroup's bytes of each dword of the plane (other workgroups
+        // write the other bytes)
+        unsigned char *dst = (unsigned char *)((unsigned *)a.lattice + (size_t)(r0 >> 5) * a.stride) + ((r0 & 31) >> 3);
+        for (int i = tid; i < N; i += blockDim.x) {
+            if constexpr (LAT == LAT_MULTI16) *(unsigned short *)(dst + 4 * (size_t)i) = ((const unsigned short *)lat_lds)[i];
+            else dst[4 * (size_t)i] = lat_lds[i];
+        }
     } else {
         if constexpr (LAT == LAT_BITS) {
             __syncthreads();

@@ -175,8 +175,9 @@ void gasp_free(void *d_ptr);
  * consecutive attempts of one replica; 0 = automatic) and waves per workgroup
  * for methods 0/1 (0 = automatic).  For method 2 the second value is the word
  * split: 1 = a workgroup holds the 64 replicas of a 64-bit site word (slots up
- * to 16), 2 = it holds one 32-bit half (32 replicas, slots up to 32); 0 =
- * automatic, which splits.
+ * to 16), 2 = it holds one 32-bit half (32 replicas, slots up to 32), 4 and
+ * 8 = a 16- or 8-bit part (16 or 8 replicas, slots up to 32); 0 = automatic:
+ * halves, or smaller parts when that leaves fewer workgroups than CUs.
  * Method 0 also takes 128, 256 or 512 slots: one replica per workgroup, spread
  * over 2, 4 or 8 waves (the waves value is then implied).  Other methods: up
  * to 64.

@@ -16,9 +16,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 BETA, H = 0.54, 0.07
-# (method, slots per replica S, second tuning value: waves for 0/1, word split for 2)
+# (method, slots per replica S, second tuning value: waves for 0/1, word split for 2:
+# 1, 2, 4, 8 = 64, 32, 16, 8 replicas per workgroup)
 # Method 0 with 128 / 256 / 512 slots spreads a replica over 2 / 4 / 8 waves (sweep_hbm_waves).
 CASES = ([(2, s, 1) for s in (1, 2, 4, 8, 16)] + [(2, s, 2) for s in (1, 4, 16, 32)] +
+         [(2, s, 4) for s in (1, 8, 32)] + [(2, s, 8) for s in (2, 16, 32)] +
          [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4), (0, 512, 8)])
 
 
@@ -253,7 +255,8 @@ def test_method2_largest_lattices(gpu, engine_lib, oracle, L, S, W):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("method,S,W,L", [(2, 16, 2, 96), (2, 32, 2, 100), (1, 16, 1, 96), (2, 8, 2, 200)])
+@pytest.mark.parametrize("method,S,W,L", [(2, 16, 2, 96), (2, 32, 2, 100), (1, 16, 1, 96), (2, 8, 2, 200),
+                                           (2, 16, 4, 96), (2, 16, 8, 200), (2, 32, 8, 100)])
 @pytest.mark.parametrize("start", ["random", "down"])
 def test_folded_filter_large_L(gpu, engine_lib, oracle, method, S, W, L, start):
     """L > 64 not a power of two: the conflict filter folds rows and columns

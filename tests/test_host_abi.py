@@ -217,8 +217,9 @@ def test_default_launch_shapes(engine_lib):
     """Launch shapes chosen without the tuner (256 CUs): method 0 runs a whole
     wave per replica from N >= 4096, and 2 or 4 waves per replica when
     N >= 16384 and the replicas alone would not fill the device; method 2 keeps
-    split words with 16 slots per replica at L=64 with 16384 replicas and never
-    goes below 8 slots (method 1: 8).  An explicit 128-slot
+    split words with 16 slots per replica at L=64 with 16384 replicas, splits
+    further (16 or 8 replicas per workgroup) when halves would leave CUs
+    without a workgroup, and never goes below 8 slots (method 1: 8).  An explicit 128-slot
     setting for method 0 implies two waves."""
     G = engine_lib
     G.set_tuning(0, 0)
@@ -230,7 +231,10 @@ def test_default_launch_shapes(engine_lib):
         assert G.get_tuning(0, 256, 16384) == (64, 1)
         assert G.get_tuning(2, 64, 16384) == (16, 2)  # split words
         assert G.get_tuning(2, 64, 524288) == (8, 2)  # at least 8 slots with replicas to spare
-        assert G.get_tuning(2, 160, 64) == (16, 2)  # split: whole words would not fit LDS at L = 160
+        assert G.get_tuning(2, 160, 64) == (16, 8)  # one group over 8 workgroups (whole words would not fit)
+        assert G.get_tuning(2, 96, 2048) == (16, 8)  # 32 groups: eighth words, 256 workgroups
+        assert G.get_tuning(2, 64, 4096) == (16, 4)  # 64 groups: quarter words
+        assert G.get_tuning(2, 64, 8192) == (16, 2)  # 128 groups: halves already fill the CUs
         assert G.get_tuning(1, 64, 524288) == (8, 1)  # method 1: at least 8
         G.set_tuning(128, 0)
         assert G.get_tuning(0, 16, 96) == (128, 2)
