@@ -330,9 +330,9 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         // LDS) ran at 2.61e11 attempts/s; split words with 8 slots run at
         // 4.44e11 (4 slots: 3.88e11, 16 slots: 4.09e11).
         if (method == 2) S = std::max(S, std::min(8, Smax));
-        // ... and at most 16, the widest group the conflict filter covers
-        // (32 unfiltered slots made the surrogate's fallback shooting, 16256
-        // replicas, 50% slower); gasp_set_tuning can still ask for 32
+        // ... and at most 16: 32-lane groups conflict more than their extra
+        // lanes gain (at L=64 with 16384 replicas 35.9 against 21.6 ms per
+        // 100-sweep launch); gasp_set_tuning can still ask for 32
         if (method == 2) S = std::min(S, 16);
         // Method 1 likewise: 8 slots ran at 3.34e11 against 2.44e11 with one.
         if (method == 1) S = std::max(S, 8);
@@ -603,9 +603,9 @@ static int engine_adapt_slots(gasp_engine *e) {
     if (!e->adapt || !e->freeze) return GASP_OK;
     const long long target = 16LL * e->ncu * 64;
     const long long active = std::max(1, e->R - e->decided);
-    // up to 16: the conflict filter covers groups of at most 16 lanes, and
-    // unfiltered 32-lane groups made the surrogate's fallback shooting 50%
-    // slower
+    // up to 16: with 32 the committor leg's shooting (524288 replicas) took
+    // 4.27 s against 3.99 and the surrogate's fallback shooting (16256)
+    // 0.48 s against 0.32, the straggler rounds conflicting more
     const int smax = 16;
     const int S = std::max(std::min(pow2_ceil((target + active - 1) / active), smax), e->S0);
     if (S == e->S) return GASP_OK;
