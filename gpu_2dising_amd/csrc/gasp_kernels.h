@@ -263,7 +263,31 @@ struct Spins {
 };
 template <int LAT>
 __device__ __forceinline__ Spins read_spins(const Lattice<LAT> &lat, const Stencil &st) {
-    return Spins{lat.spin(st.c), lat.spin(st.up), lat.spin(st.dn), lat.spin(st.lt), lat.spin(st.rt)};
+    if constexpr (LAT == LAT_HBM) {
+        // The centre row's bytes lo .. hi (c - 1 .. c + 1 where the row does
+        // not wrap) come from the aligned dword holding lo and, for the lanes
+        // whose hi lies in the next one, that dword too (hi < N, so it is in
+        // the lattice): fewer scattered loads per attempt, which is what the
+        // byte lattice's rounds wait on.  A wrapped neighbour is read alone.
+        const int c = st.c;
+        const bool wl = st.lt != c - 1, wr = st.rt != c + 1;  // column 0 / L - 1
+        const int lo = wl ? c : c - 1, hi = wr ? c : c + 1;
+        const int d0 = lo & ~3;
+        const unsigned *dw = (const unsigned *)lat.base;
+        const unsigned v0 = __hip_atomic_load(dw + (d0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        unsigned v1 = 0u;
+        if ((hi & ~3) != d0)
+            v1 = __hip_atomic_load(dw + (d0 >> 2) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const int up = lat.spin(st.up), dn = lat.spin(st.dn);
+        const int lt_far = wl ? lat.spin(st.lt) : 0, rt_far = wr ? lat.spin(st.rt) : 0;
+        auto byte_at = [&](int x) {
+            const unsigned v = (x & ~3) == d0 ? v0 : v1;
+            return (int)((v >> (8 * (x & 3))) & 1u);
+        };
+        return Spins{byte_at(c), up, dn, wl ? lt_far : byte_at(c - 1), wr ? rt_far : byte_at(c + 1)};
+    } else {
+        return Spins{lat.spin(st.c), lat.spin(st.up), lat.spin(st.dn), lat.spin(st.lt), lat.spin(st.rt)};
+    }
 }
 // acc: the draw accepts the flip (whether the attempt is enabled is the caller's)
 __device__ __forceinline__ Attempt decide(const Stencil &st, const Spins &x, uint32_t ua) {
