@@ -306,7 +306,7 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     } else {
         W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
     }
-    // method 0 may spread a replica over 2 or 4 waves (sweep_hbm_waves)
+    // method 0 may spread a replica over 2, 4 or 8 waves (sweep_hbm_waves)
     int Smax = (method == 2) ? (W >= 2 ? 32 : 16) : (method == 0 ? 512 : 64);
     int S;
     if (g_tune_slots > 0) {

@@ -5,7 +5,7 @@
 // streams (counter-based Philox4x32-10, or a pre-generated per-replica MT19937
 // draw stream that reproduces src/mc_cpu.c:41-89), and a speculation width S
 // up to one wave; sweep_hbm_waves extends layout 0 to a replica spread over
-// two or four waves of a workgroup (S = 128, 256).
+// two, four or eight waves of a workgroup (S = 128, 256, 512).
 //
 // Dynamics (identical in every layout).  Attempt a of a replica picks a site,
 // reads the spin s and its four periodic neighbours, forms

@@ -104,9 +104,10 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
 
 @pytest.mark.parametrize("S", [128, 256, 512])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
-    """A replica spread over 2 or 4 waves at the batched run's L = 256 shape,
-    from all -1 at the reference's (beta, h) and from a random start at
-    beta = 0.3 (about half the attempts accepted)."""
+    """A replica spread over 2, 4 or 8 waves at the batched run's L = 256
+    shape (R = 6, so the conflict table has fewer entries than sites and
+    sites share entries), from all -1 at the reference's (beta, h) and from
+    a random start at beta = 0.3 (about half the attempts accepted)."""
     G, O = engine_lib, oracle
     L, R, seed = 256, 6, 4242
     for start, beta in (("down", BETA), ("random", 0.3)):

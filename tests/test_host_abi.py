@@ -215,7 +215,7 @@ print("OK")
 
 def test_default_launch_shapes(engine_lib):
     """Launch shapes chosen without the tuner (256 CUs): method 0 runs a whole
-    wave per replica from N >= 4096, and 2 or 4 waves per replica when
+    wave per replica from N >= 4096, and 2 or 4 waves per replica (8 only from the tuner) when
     N >= 16384 and the replicas alone would not fill the device; method 2 keeps
     split words with 16 slots per replica at L=64 with 16384 replicas, splits
     further (16 or 8 replicas per workgroup) when halves would leave CUs
