@@ -280,10 +280,9 @@ __device__ __forceinline__ Spins read_spins(const Lattice<LAT> &lat, const Stenc
             v1 = __hip_atomic_load(dw + (d0 >> 2) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const int up = lat.spin(st.up), dn = lat.spin(st.dn);
         const int lt_far = wl ? lat.spin(st.lt) : 0, rt_far = wr ? lat.spin(st.rt) : 0;
-        auto byte_at = [&](int x) {
-            const unsigned v = (x & ~3) == d0 ? v0 : v1;
-            return (int)((v >> (8 * (x & 3))) & 1u);
-        };
+        // the window lo .. hi lies within the 8 bytes from d0: one 64-bit shift per byte
+        const uint64_t v = ((uint64_t)v1 << 32) | v0;
+        auto byte_at = [&](int x) { return (int)((v >> (8 * (x - d0))) & 1ull); };
         return Spins{byte_at(c), up, dn, wl ? lt_far : byte_at(c - 1), wr ? rt_far : byte_at(c + 1)};
     } else {
         return Spins{lat.spin(st.c), lat.spin(st.up), lat.spin(st.dn), lat.spin(st.lt), lat.spin(st.rt)};
