@@ -119,6 +119,21 @@ def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
         assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
+def test_hbm_waves_small_table(gpu, engine_lib, oracle):
+    """600 replicas of L = 256 over four waves each: the conflict table is
+    sized for three workgroups per CU (8192 entries on a 160 KB-LDS device),
+    so eight sites share each entry and spurious conflicts are common, which
+    must only delay attempts, never change them."""
+    G, O = engine_lib, oracle
+    L, R, seed = 256, 600, 777
+    for start, beta in (("down", BETA), ("random", 0.3)):
+        print(f"[L256 small table] start={start}", flush=True)
+        g0 = _start(start, R, L, seed)
+        got, mag = _run_engine(G, L, R, 0, 256, g0, seed, False, beta, H, [1], W=4)
+        ref = O.sweep_philox(L, g0.copy(), seed, 0, L * L, beta, H)
+        assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("L", [8, 16, 33, 64])
 def test_largest_cluster(gpu, engine_lib, oracle, L):
     G, O = engine_lib, oracle

@@ -139,6 +139,20 @@ def test_bootstrap_libc_batch_is_sequential_libc(engine_lib, seed):
         assert np.array_equal(got[~np.isnan(ref)], ref[~np.isnan(ref)]), nthreads
 
 
+def test_bootstrap_libc_edges(engine_lib):
+    """No inputs; one fate per input (n = 1: every draw is index 0); more
+    threads than inputs; an input with no decided fate gives NaN, as the
+    reference's 0/0."""
+    G = engine_lib
+    assert G.bootstrap_libc(np.zeros((0, 5), dtype=np.int32), 3).shape == (0,)
+    one = G.bootstrap_libc(np.array([[1], [0], [-1]], dtype=np.int32), 7, nthreads=16)
+    assert one[0] == 0.0 and one[1] == 0.0 and np.isnan(one[2])
+    f = np.array([[0, 1, 1, 0, 1]] * 3, dtype=np.int32)
+    a = G.bootstrap_libc(f, 50, seed=3, nthreads=1)
+    b = G.bootstrap_libc(f, 50, seed=3, nthreads=64)
+    assert np.array_equal(a, b) and np.all(a > 0)
+
+
 def test_threshold_table_layout():
     """Kernel table order j = 5*s01 + (up neighbours) is the reference index
     5(s+1) + nsum + 4 = 10 s01 + 2c (s = 2 s01 - 1, nsum = 2c - 4)."""
