@@ -302,6 +302,9 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
             const long long groups = (R + 63) / 64;
             W = 2;
             while (W < 8 && groups * W < ncu) W *= 2;
+            // ... and until the lattice fits the LDS (eighth words: a byte
+            // per site, L up to about 400 on 160 KB)
+            while (W < 8 && lds_bytes_for(2, N, 1, W, R) > lds_block) W *= 2;
         }
     } else {
         W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;

@@ -177,7 +177,8 @@ void gasp_free(void *d_ptr);
  * split: 1 = a workgroup holds the 64 replicas of a 64-bit site word (slots up
  * to 16), 2 = it holds one 32-bit half (32 replicas, slots up to 32), 4 and
  * 8 = a 16- or 8-bit part (16 or 8 replicas, slots up to 32); 0 = automatic:
- * halves, or smaller parts when that leaves fewer workgroups than CUs.
+ * halves, or smaller parts when that leaves fewer workgroups than CUs or
+ * when the lattice would not fit the LDS otherwise.
  * Method 0 also takes 128, 256 or 512 slots: one replica per workgroup, spread
  * over 2, 4 or 8 waves (the waves value is then implied).  Other methods: up
  * to 64.

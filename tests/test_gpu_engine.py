@@ -272,7 +272,8 @@ def test_method2_largest_lattices(gpu, engine_lib, oracle, L, S, W):
 
 
 @pytest.mark.parametrize("method,S,W,L", [(2, 16, 2, 96), (2, 32, 2, 100), (1, 16, 1, 96), (2, 8, 2, 200),
-                                           (2, 16, 4, 96), (2, 16, 8, 200), (2, 32, 8, 100)])
+                                           (2, 16, 4, 96), (2, 16, 8, 200), (2, 32, 8, 100),
+                                           (2, 16, 8, 300)])  # eighth words past halves' L = 201
 @pytest.mark.parametrize("start", ["random", "down"])
 def test_folded_filter_large_L(gpu, engine_lib, oracle, method, S, W, L, start):
     """L > 64 not a power of two: the conflict filter folds rows and columns

@@ -249,6 +249,9 @@ def test_default_launch_shapes(engine_lib):
         assert G.get_tuning(2, 96, 2048) == (16, 8)  # 32 groups: eighth words, 256 workgroups
         assert G.get_tuning(2, 64, 4096) == (16, 4)  # 64 groups: quarter words
         assert G.get_tuning(2, 64, 8192) == (16, 2)  # 128 groups: halves already fill the CUs
+        # many replicas of a large lattice: the split grows until a workgroup's
+        # lattice fits the LDS (64 KB assumed without a GPU: eighth words at L = 250)
+        assert G.get_tuning(2, 250, 16384) == (16, 8)
         assert G.get_tuning(1, 64, 524288) == (8, 1)  # method 1: at least 8
         G.set_tuning(128, 0)
         assert G.get_tuning(0, 16, 96) == (128, 2)
