@@ -201,9 +201,16 @@ def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
     attempts = sum(Lg * Lg * Rg for (Lg, Rg, _) in BATCH_GROUPS) * chk
     # branches captured longest first (each group's time alone estimated from
     # its tuner probe rate), so that the long groups start before the short
-    # ones fill the CUs: 9.6 against 10.4 ms per replay in capture order
+    # ones fill the CUs (9.6 against 10.4 ms per replay in capture order);
+    # except that an HBM-path group whose replicas each take a workgroup with
+    # a large LDS conflict table does not go first: dispatched first, those
+    # workgroups kept the next group's small ones from spreading evenly over
+    # the CUs (7.47 against 6.75 ms per replay with the L = 96 group first)
     est = [Lg * Lg * Rg / max(t["probe_rate"], 1.0) for (Lg, Rg, _), t in zip(BATCH_GROUPS, tuned)]
     order = sorted(range(len(engines)), key=lambda i: -est[i])
+    big_lds = [m == 0 and t["slots"] > 64 for (_, _, m), t in zip(BATCH_GROUPS, tuned)]
+    if len(order) > 1 and big_lds[order[0]] and not big_lds[order[1]]:
+        order[0], order[1] = order[1], order[0]
     graph = G.Graph([engines[i] for i in order], [chk] * len(engines), [checks[i] for i in order],
                     [dn[i] for i in order], [up[i] for i in order])
     graph.launch(1)  # warm-up
