@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -270,15 +271,21 @@ extern "C" int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, 
     gasp::clear_error();
     if (!fates || !out || ninputs < 0 || n < 1 || m < 1)
         return gasp::set_error(GASP_ERR_ARG, "gasp_bootstrap_errbar_libc: bad argument");
+    constexpr int kMaxThreads = 256;
     if (nthreads < 1) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-    nthreads = std::max(1, std::min(nthreads, ninputs));
+    nthreads = std::max(1, std::min({nthreads, ninputs, kMaxThreads}));
     const uint64_t per_input = (uint64_t)n * (uint64_t)m;  // rand() calls of one input
+    std::atomic<bool> failed{false};
     auto work = [&](int i0, int i1) {
-        uint32_t w[31];
-        libc_window(seed, (uint64_t)i0 * per_input, w);
-        std::vector<uint32_t> buf;
-        std::vector<double> store((size_t)m);
-        for (int i = i0; i < i1; ++i) out[i] = bootstrap_one(fates + (size_t)i * n, n, m, w, buf, store);
+        try {  // an exception must not leave a thread (std::terminate)
+            uint32_t w[31];
+            libc_window(seed, (uint64_t)i0 * per_input, w);
+            std::vector<uint32_t> buf;
+            std::vector<double> store((size_t)m);
+            for (int i = i0; i < i1; ++i) out[i] = bootstrap_one(fates + (size_t)i * n, n, m, w, buf, store);
+        } catch (...) {
+            failed = true;
+        }
     };
     std::vector<std::thread> pool;
     const int chunk = (ninputs + nthreads - 1) / nthreads;
@@ -293,6 +300,7 @@ extern "C" int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, 
     }
     work(0, std::min(ninputs, chunk));
     for (auto &th : pool) th.join();
+    if (failed) return gasp::set_error(GASP_ERR_NOMEM, "gasp_bootstrap_errbar_libc: out of host memory");
     return 0;
 }
 

@@ -275,7 +275,8 @@ int gasp_bootstrap_errbar_gpu(const int *fates, int on_device, int ninputs, int 
  * bootstrap_errbar(n, m, ...) after srand(seed) with glibc's rand() (seed 1 is
  * glibc's unseeded state): the library generates glibc's rand() sequence
  * itself and jumps it ahead to each input's first draw, so the inputs run on
- * `nthreads` host threads (0: one per core).  Does not touch the process's
+ * `nthreads` host threads (0: one per core; at most 256, and no more than
+ * ninputs).  Does not touch the process's
  * own rand() state.  out[ninputs]. */
 int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, int m, unsigned seed, int nthreads,
                                double *out);

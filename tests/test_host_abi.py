@@ -141,8 +141,8 @@ def test_bootstrap_libc_batch_is_sequential_libc(engine_lib, seed):
 
 def test_bootstrap_libc_edges(engine_lib):
     """No inputs; one fate per input (n = 1: every draw is index 0); more
-    threads than inputs; an input with no decided fate gives NaN, as the
-    reference's 0/0."""
+    threads than inputs, or than the thread cap; an input with no decided
+    fate gives NaN, as the reference's 0/0."""
     G = engine_lib
     assert G.bootstrap_libc(np.zeros((0, 5), dtype=np.int32), 3).shape == (0,)
     one = G.bootstrap_libc(np.array([[1], [0], [-1]], dtype=np.int32), 7, nthreads=16)
@@ -151,6 +151,11 @@ def test_bootstrap_libc_edges(engine_lib):
     a = G.bootstrap_libc(f, 50, seed=3, nthreads=1)
     b = G.bootstrap_libc(f, 50, seed=3, nthreads=64)
     assert np.array_equal(a, b) and np.all(a > 0)
+    # a thread count far above the cap of 256 threads (300 inputs)
+    f = np.random.default_rng(5).integers(-1, 2, size=(300, 40), dtype=np.int32)
+    a = G.bootstrap_libc(f, 20, seed=9, nthreads=1)
+    b = G.bootstrap_libc(f, 20, seed=9, nthreads=100000)
+    assert np.array_equal(a, b, equal_nan=True)
 
 
 def test_threshold_table_layout():
