@@ -318,8 +318,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--method", type=int, default=2)
     ap.add_argument("--slots", type=int, default=0, help="slots per replica (0 = tuner)")
-    ap.add_argument("--word-split", type=int, default=0, choices=(0, 1, 2),
-                    help="replicas per workgroup: 1 = 64 (whole words), 2 = 32 (split words), 0 = tuner")
+    ap.add_argument("--word-split", type=int, default=0, choices=(0, 1, 2, 4, 8),
+                    help="replicas per workgroup: 1 = 64 (whole words), 2 = 32, 4 = 16, 8 = 8 (split words), "
+                         "0 = tuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dump", action="store_true", help="skip the grid dumps")
     ap.add_argument("--no-committor", action="store_true", help="skip the committor shooting leg")
@@ -374,7 +375,8 @@ def main() -> int:
     bpa = HBM_BYTES_PER_ATTEMPT if hbm else LDS_BYTES_PER_ATTEMPT
     peak = HBM_PEAK_GBS if hbm else LDS_PEAK_GBS
     achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
-    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: "LAT_MULTIHALF" if split == 2 else "LAT_MULTISPIN"}[args.method]
+    multi = {1: "LAT_MULTISPIN", 2: "LAT_MULTIHALF", 4: "LAT_MULTI16", 8: "LAT_MULTI8"}
+    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: multi.get(split, "LAT_MULTISPIN")}[args.method]
     sweeps = np.arange(1, args.steps + 1) * SWEEPS_PER_STEP
     k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
 
@@ -399,7 +401,7 @@ def main() -> int:
             "L": L, "replicas": R, "sweeps_per_step": SWEEPS_PER_STEP,
             "total_sweeps": args.steps * SWEEPS_PER_STEP, "gpu_method": args.method,
             "rng": "philox4x32-10 per replica", "slots_per_replica": S,
-            "replicas_per_workgroup": (32 if split == 2 else 64) if args.method == 2 else None,
+            "replicas_per_workgroup": 64 // max(split, 1) if args.method == 2 else None,
             "grid_dumps": not args.no_dump,
         },
         "roofline": {
