@@ -160,83 +160,42 @@ def surrogate_leg(G, shooting: dict, n_inputs: int = 512, fallback_reps: int = 1
     }
 
 
-BATCH_GROUPS = [(32, 8192, 2), (48, 4096, 2), (64, 4096, 2), (96, 2048, 2), (256, 256, 0)]
-
-
 def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
     """Uneven batch in one captured graph: L in {32, 48, 64, 96} (multispin,
     LDS) with {8192, 4096, 4096, 2048} replicas and L = 256 (HBM path) with
     256; beta ~ U[0.50, 0.60], h ~ U[0.03, 0.10] per group; itask 0 (count
     nucleated) and itask 1 (fates from droplet inputs) alternate.  Launch
-    shapes come from the autotuner.  One replay = `chk` sweeps of every group
-    plus its threshold check."""
-    rs = np.random.default_rng(seed)
-    tuned = []
-    for (Lg, Rg, m) in BATCH_GROUPS:
-        s, w, rate = G.autotune(m, Lg, Rg, 4)
-        tuned.append({"L": Lg, "replicas": Rg, "method": m, "slots": s, "waves": w, "probe_rate": rate})
-    engines, rngs, checks, dn, up = [], [], [], [], []
-    for k, (Lg, Rg, m) in enumerate(BATCH_GROUPS):
-        beta, h = rs.uniform(0.50, 0.60), rs.uniform(0.03, 0.10)
-        tuned[k].update(beta=beta, h=h, itask=k % 2)
-        rng = G.RngState(Rg, seed + k)
-        e = G.Engine(Lg, Rg, m, rng, beta, h)
-        if k % 2 == 0:
-            e.set_grids(-np.ones((Rg, Lg, Lg), dtype=np.int32))
-            checks.append(G.CHECK_COUNT)
-        else:
-            r = np.arange(Lg)
-            ins = []
-            for j in range(16):  # droplets of radius L/10 .. L/5
-                rad = Lg / 10 + (Lg / 10) * j / 15
-                d2 = (r[:, None] - Lg // 2) ** 2 + (r[None, :] - Lg // 2) ** 2
-                ins.append(np.where(d2 <= rad * rad, 1, -1))
-            e.set_inputs(np.stack(ins).astype(np.int32))
-            e.set_fates(np.full(Rg, -1, dtype=np.int32))
-            checks.append(G.CHECK_FATES)
-        dn.append(-0.9)
-        up.append(0.9)
-        engines.append(e)
-        rngs.append(rng)
-    attempts = sum(Lg * Lg * Rg for (Lg, Rg, _) in BATCH_GROUPS) * chk
-    # branches captured longest first (each group's time alone estimated from
-    # its tuner probe rate), so that the long groups start before the short
-    # ones fill the CUs (9.6 against 10.4 ms per replay in capture order);
-    # except that an HBM-path group whose replicas each take a workgroup with
-    # a large LDS conflict table does not go first: dispatched first, those
-    # workgroups kept the next group's small ones from spreading evenly over
-    # the CUs (7.47 against 6.75 ms per replay with the L = 96 group first)
-    est = [Lg * Lg * Rg / max(t["probe_rate"], 1.0) for (Lg, Rg, _), t in zip(BATCH_GROUPS, tuned)]
-    order = sorted(range(len(engines)), key=lambda i: -est[i])
-    big_lds = [m == 0 and t["slots"] > 64 for (_, _, m), t in zip(BATCH_GROUPS, tuned)]
-    if len(order) > 1 and big_lds[order[0]] and not big_lds[order[1]]:
-        order[0], order[1] = order[1], order[0]
-    graph = G.Graph([engines[i] for i in order], [chk] * len(engines), [checks[i] for i in order],
-                    [dn[i] for i in order], [up[i] for i in order])
+    shapes and sweeps per launch come from the autotuner, probed at each
+    group's own (beta, h).  One replay = `chk` sweeps of every group plus its
+    threshold check."""
+    from gpu_2dising_amd.batched import Batch
+
+    b = Batch(chk=chk, seed=seed)
+    attempts = b.attempts_per_replay
+    graph = b.graph
     graph.launch(1)  # warm-up
     ms0 = graph.ms()
     t0 = time.perf_counter()
     graph.launch(replays)
     wall = time.perf_counter() - t0
     g_ms = (graph.ms() - ms0) / replays
-    graph.close()
     # the same work one group after another on the engines' own streams
     t0 = time.perf_counter()
     for _ in range(replays):
-        for e, c, d, u in zip(engines, checks, dn, up):
-            e.sweep(chk, c, d, u)
-            e.sync()
+        for g in b.groups:
+            g.engine.sweep(chk, G.CHECK_COUNT if g.itask == 0 else G.CHECK_FATES, b.dn, b.up)
+            g.engine.sync()
     seq_ms = (time.perf_counter() - t0) * 1e3 / replays
-    for e in engines:
-        e.close()
-    for r in rngs:
-        r.free()
+    groups = [{"L": g.L, "replicas": g.R, "method": g.method, "slots": g.slots, "waves": g.waves,
+               "sweeps_per_launch": g.launch_sweeps, "probe_rate": g.probe_rate, "beta": g.beta, "h": g.h,
+               "itask": g.itask} for g in b.groups]
+    order = [b.groups[i].L for i in b.order]
+    b.close()
     return {"metric": "batched spin-flip attempts/sec (one captured graph, uneven L)",
             "value": attempts / (g_ms * 1e-3), "unit": "attempts/s",
             "wall_value": attempts * replays / wall, "graph_ms_per_replay": g_ms,
             "sequential_ms_per_replay": seq_ms, "sweeps_per_replay": chk, "replays": replays,
-            "attempts_per_replay": attempts, "capture_order_L": [BATCH_GROUPS[i][0] for i in order],
-            "groups": tuned}
+            "attempts_per_replay": attempts, "capture_order_L": order, "groups": groups}
 
 
 def fit_rate(sweeps: np.ndarray, frac: np.ndarray):

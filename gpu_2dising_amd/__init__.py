@@ -26,6 +26,8 @@ EXPORTS = (
     "gpuInitDevice", "gpuInitGrid", "gpuInitRand", "preComputeProbs_gpu", "select_gpu_method",
     "mc_driver_gpu", "init_grids_uniform", "init_grids_from_file", "init_grids_from_array",
     "init_fates", "read_input_grid", "write_ising_grids", "bootstrap_errbar",
+    "init_genrand", "genrand_int32", "genrand_real3", "preComputeProbs_cpu", "mc_sweep_cpu",
+    "compute_magnetisation_cpu", "compute_largest_cluster_cpu", "mc_driver_cpu",
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
@@ -163,7 +165,8 @@ def lib() -> C.CDLL:
     L.gasp_graph_ms.restype = C.c_double
     L.gasp_graph_destroy.argtypes = [vp]
     L.gasp_graph_destroy.restype = C.c_int
-    L.gasp_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp]
+    L.gasp_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, ip, ip, ip,
+                                dp]
     L.gasp_autotune.restype = C.c_int
     L.gasp_bootstrap_errbar_gpu.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, vp]
     L.gasp_bootstrap_errbar_gpu.restype = C.c_int
@@ -412,10 +415,13 @@ class Graph:
             self.ptr = C.c_void_p()
 
 
-def autotune(method: int, L: int, ngrids: int, probe_sweeps: int = 5):
-    s, w, r = C.c_int(), C.c_int(), C.c_double()
-    check(lib().gasp_autotune(method, L, ngrids, probe_sweeps, C.byref(s), C.byref(w), C.byref(r)))
-    return s.value, w.value, r.value
+def autotune(method: int, L: int, ngrids: int, probe_sweeps: int = 5, beta: float = 0.54, h: float = 0.07,
+             check_sweeps: int = 0):
+    """gasp_autotune: (slots, waves or word split, sweeps per launch, attempts/s)."""
+    s, w, k, r = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    check(lib().gasp_autotune(method, L, ngrids, beta, h, probe_sweeps, check_sweeps, C.byref(s), C.byref(w),
+                              C.byref(k), C.byref(r)))
+    return s.value, w.value, k.value, r.value
 
 
 def bootstrap_gpu(fates: np.ndarray, m: int, seed: int = 1) -> np.ndarray:

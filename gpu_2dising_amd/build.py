@@ -53,11 +53,13 @@ def build_lib() -> str:
     eng_o = os.path.join(BUILD, "gasp_engine.o")
     cnn_o = os.path.join(BUILD, "gasp_cnn.o")
     host_o = os.path.join(BUILD, "gasp_host.o")
+    cpu_o = os.path.join(BUILD, "gasp_cpu.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gasp_engine.hip"), "-o", eng_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gasp_cnn.hip"), "-o", cnn_o])
     _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "gasp_host.cpp"), "-o", host_o])
+    _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "gasp_cpu.cpp"), "-o", cpu_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, cnn_o, host_o])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, cnn_o, host_o, cpu_o])
     os.replace(tmp, LIB)
     return LIB
 

@@ -10,8 +10,9 @@
 //   GASP_RNG    "philox" (default) or "mt19937" (per-replica MT19937 streams,
 //               replica r seeded GASP_SEED + r: the reference CPU stream)
 //   GASP_L      lattice size (the reference hard-codes 64)
-// There is no CPU fallback: without a visible GPU the program exits with an
-// error.
+// Without a visible GPU it falls back to the reference's CPU engine
+// (mc_driver_cpu, one MT19937 stream seeded with the seed), as
+// src/ising.cu:127-155 does; GASP_RNG then has no effect.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -71,12 +72,43 @@ int main(int argc, char *argv[]) {
         return EXIT_FAILURE;
     }
 
+    // the reference passes a single float here although itask 0 fills
+    // tot_nsweeps/mag_output_int values
+    const int nres = itask == 0 ? (mag_output_int > 0 ? tot_nsweeps / mag_output_int : 0) : 1;
+    std::vector<float> result(nres > 0 ? nres : 1, 0.0f);
+    char cv[] = "magnetisation";
+    char fname[] = "gridstates.bin";
+    mc_sampler_t samples;
+    samples.tot_nsweeps = tot_nsweeps;
+    samples.mag_output_int = mag_output_int;
+    samples.grid_output_int = grid_output_int;
+    mc_function_t calc;
+    calc.itask = itask;
+    calc.dn_thr = dn_threshold;
+    calc.up_thr = up_threshold;
+    calc.ninputs = 1;
+    calc.initial_spin = -1;
+    calc.cv = cv;
+    calc.result = result.data();
+    calc.filename = fname;
+
     int gpu_nsms = 0;
     if (gpuInitDevice(gpu_device, &gpu_nsms) == -1) {
-        fprintf(stderr, "gasp: no GPU visible; this build has no CPU fallback\n");
+        // src/ising.cu:131-155
+        printf("Falling back to CPU\n");
+        fflush(stdout);
+        gasp_clear_error();
+        init_genrand(rngseed);
+        preComputeProbs_cpu(beta, h);
+        mc_grids_t cgrids;
+        cgrids.L = L;
+        cgrids.ngrids = ngrids;
+        cgrids.ising_grids = ising_grids;
+        mc_driver_cpu(cgrids, beta, h, grid_fate, samples, calc, write_ising_grids);
+        const int rc = gasp_last_error();
         free(ising_grids);
         free(grid_fate);
-        return EXIT_FAILURE;
+        return rc ? EXIT_FAILURE : EXIT_SUCCESS;
     }
 
     int *d_ising_grids = nullptr, *d_neighbour_list = nullptr;
@@ -92,32 +124,12 @@ int main(int argc, char *argv[]) {
     }
     preComputeProbs_gpu(beta, h);
 
-    // the reference passes a single float here although itask 0 fills
-    // tot_nsweeps/mag_output_int values
-    const int nres = itask == 0 ? (mag_output_int > 0 ? tot_nsweeps / mag_output_int : 0) : 1;
-    std::vector<float> result(nres > 0 ? nres : 1, 0.0f);
-    char cv[] = "magnetisation";
-    char fname[] = "gridstates.bin";
-
     mc_gpu_grids_t grids;
     grids.L = L;
     grids.ngrids = ngrids;
     grids.ising_grids = ising_grids;
     grids.d_ising_grids = d_ising_grids;
     grids.d_neighbour_list = d_neighbour_list;
-    mc_sampler_t samples;
-    samples.tot_nsweeps = tot_nsweeps;
-    samples.mag_output_int = mag_output_int;
-    samples.grid_output_int = grid_output_int;
-    mc_function_t calc;
-    calc.itask = itask;
-    calc.dn_thr = dn_threshold;
-    calc.up_thr = up_threshold;
-    calc.ninputs = 1;
-    calc.initial_spin = -1;
-    calc.cv = cv;
-    calc.result = result.data();
-    calc.filename = fname;
     gpu_run_t gpu_state;
     gpu_state.d_state = d_state;
     gpu_state.threadsPerBlock = threadsPerBlock;

@@ -148,6 +148,13 @@ DeviceInfo device_info() {
     return d;
 }
 
+// src/gpu_tools.cu:212-224 with the shared memory per block as an argument.
+int select_method_for(int L, long long shmem, int tpb) {
+    const int m = (int)sqrt((8.0 * (double)shmem) / (double)tpb);
+    const int maxL = (int)sqrt((double)(32 * ((m * m) / 32)));
+    return L <= maxL ? 2 : 0;
+}
+
 }  // namespace
 
 // ============================================================ random streams
@@ -1236,6 +1243,14 @@ extern "C" int gasp_graph_create(gasp_engine **engines, int n, const int *nsweep
 extern "C" int gasp_graph_launch(gasp_graph *g, int replays) {
     clear_error();
     if (!g || replays < 0) return set_error(GASP_ERR_ARG, "gasp_graph_launch: bad argument");
+    // Direct sweeps on an engine since the last replay advanced only its host
+    // counter, and may still be running on its stream: wait for them, then
+    // hand the counter to the device header that the graph's kernels read.
+    for (gasp_engine *e : g->eng) {
+        const int rc = engine_sync(e);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(e->d_rng, &e->rng, sizeof(RngHeader), hipMemcpyHostToDevice, g->st));
+    }
     HIP_TRY(hipEventRecord(g->t0, g->st));
     for (int r = 0; r < replays; ++r) HIP_TRY(hipGraphLaunch(g->exec, g->st));
     HIP_TRY(hipEventRecord(g->t1, g->st));
@@ -1252,14 +1267,35 @@ extern "C" double gasp_graph_ms(gasp_graph *g) { return g ? g->ms : 0.0; }
 
 // ============================================================ autotuner
 // Times each candidate launch shape on a throw-away engine (all spins -1,
-// Philox, probe_sweeps sweeps after as many warm-up sweeps) and keeps the
-// fastest for (method, L, ngrids).  Candidates: slots per replica S (lanes
-// that speculate on one replica) and, for methods 0/1, waves per workgroup W;
-// for method 0 also 128, 256 and 512 slots (a replica over 2, 4 or 8 waves).
-extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
+// Philox, the caller's beta and h, probe_sweeps sweeps after as many warm-up
+// sweeps) and keeps the fastest for (method, L, ngrids).  Candidates: slots
+// per replica S (lanes that speculate on one replica) and, for methods 0/1,
+// waves per workgroup W (for method 2, the word split); for method 0 also
+// 128, 256 and 512 slots (a replica over 2, 4 or 8 waves).  Then, with the
+// chosen shape and check_sweeps > 0, the sweeps per launch: check_sweeps
+// sweeps as one launch or as 2, 4, ... equal launches (launches of at least
+// check_sweeps / 8 sweeps), each timed over one check interval after one of
+// warm-up; the fastest gives *best_launch_sweeps.
+static double probe_rate(gasp_engine *e, const std::vector<int> &grids, int ngrids, int sweeps, int per_launch) {
+    auto run = [&]() {
+        for (int done = 0; done < sweeps; done += per_launch)
+            if (engine_advance(e, (uint64_t)per_launch * e->N, 0, 0, 0)) return false;
+        return engine_sync(e) == GASP_OK;
+    };
+    // warm-up as long as the timed part: from the all-down start the first
+    // sweeps accept least, which flatters wide speculation groups
+    if (engine_set_grids_impl(e, grids.data(), 0, ngrids) || !run()) return -1.0;
+    const double ms0 = e->kernel_ms;
+    if (!run()) return -1.0;
+    const double ms = e->kernel_ms - ms0;
+    return ms > 0 ? (double)sweeps * e->N * ngrids / (ms * 1e-3) : -1.0;
+}
+
+extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double h, int probe_sweeps,
+                             int check_sweeps, int *best_slots, int *best_waves, int *best_launch_sweeps,
                              double *best_rate) {
     clear_error();
-    if (method < 0 || method > 2 || L < 2 || ngrids < 1 || probe_sweeps < 1)
+    if (method < 0 || method > 2 || L < 2 || ngrids < 1 || probe_sweeps < 1 || check_sweeps < 0)
         return set_error(GASP_ERR_ARG, "gasp_autotune: bad argument");
     const int save_s = g_tune_slots, save_w = g_tune_waves;
     std::vector<std::pair<int, int>> cand;
@@ -1289,7 +1325,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
     std::vector<int> grids((size_t)ngrids * L * L, -1);
     float pf[20];
     uint64_t thr[10];
-    table_f32(0.54, 0.07, pf);  // the reference's metastable regime
+    table_f32(beta, h, pf);
     thr_from_table_f(pf, thr);
     double best = -1.0;
     int bs = 1, bw = 1;
@@ -1299,7 +1335,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
         g_tune_slots = S;
         g_tune_waves = W;
         gasp_engine *e = nullptr;
-        if (engine_new(L, ngrids, method, rng, thr, 0, 0.54, 0.07, &e)) {
+        if (engine_new(L, ngrids, method, rng, thr, 0, beta, h, &e)) {
             clear_error();
             continue;
         }
@@ -1307,22 +1343,32 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
             engine_free(e);
             continue;
         }
-        double rate = -1.0;
-        // warm-up as long as the timed part: from the all-down start the
-        // first sweeps accept least, which flatters wide speculation groups
-        if (!engine_set_grids_impl(e, grids.data(), 0, ngrids) &&
-            !engine_advance(e, (uint64_t)probe_sweeps * e->N, 0, 0, 0) && !engine_sync(e)) {
-            const double ms0 = e->kernel_ms;
-            if (!engine_advance(e, (uint64_t)probe_sweeps * e->N, 0, 0, 0) && !engine_sync(e)) {
-                const double ms = e->kernel_ms - ms0;
-                if (ms > 0) rate = (double)probe_sweeps * e->N * ngrids / (ms * 1e-3);
-            }
-        }
+        const double rate = probe_rate(e, grids, ngrids, probe_sweeps, probe_sweeps);
         engine_free(e);
         if (rate > best) {
             best = rate;
             bs = S;
             bw = W;
+        }
+    }
+    int blaunch = check_sweeps;
+    if (best > 0 && check_sweeps > 1) {
+        g_tune_slots = bs;
+        g_tune_waves = bw;
+        gasp_engine *e = nullptr;
+        if (!engine_new(L, ngrids, method, rng, thr, 0, beta, h, &e)) {
+            double best_l = -1.0;
+            for (int k = 1; check_sweeps / k >= std::max(1, check_sweeps / 8); k *= 2) {
+                if (check_sweeps % k) continue;
+                const double rate = probe_rate(e, grids, ngrids, check_sweeps, check_sweeps / k);
+                if (rate > best_l) {
+                    best_l = rate;
+                    blaunch = check_sweeps / k;
+                }
+            }
+            engine_free(e);
+        } else {
+            clear_error();
         }
     }
     g_tune_slots = save_s;
@@ -1332,6 +1378,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, in
     g_tuned[std::make_tuple(method, L, ngrids)] = {bs, bw};
     if (best_slots) *best_slots = bs;
     if (best_waves) *best_waves = bw;
+    if (best_launch_sweeps) *best_launch_sweeps = blaunch;
     if (best_rate) *best_rate = best;
     return GASP_OK;
 }
@@ -1410,16 +1457,24 @@ extern "C" void preComputeProbs_gpu(double beta, double h) {
     thr_from_table_f(g_pacc_f32, g_thr_f32);
 }
 
+// src/gpu_tools.cu:205-232: the largest L whose multispin lattice the
+// reference fits in one block's shared memory, maxL = floor(sqrt(32 *
+// floor(m^2 / 32))) with m = floor(sqrt(8 shmem / tpb)) (201 at 32 threads
+// per block and 142 at 64 on a 160 KB device); 2 up to maxL, else 0.  The
+// engine's method 2 itself runs lattices beyond maxL (split words), so the
+// choice is only the reference's.
 extern "C" int select_gpu_method(int L, int ngrids, int threadsPerBlock, int gpu_device) {
     (void)ngrids;
-    (void)threadsPerBlock;
+    if (threadsPerBlock < 1) {
+        set_error(GASP_ERR_ARG, "select_gpu_method: threadsPerBlock must be positive");
+        return 0;
+    }
+    int dev = gpu_device;
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) dev = 0;
     hipDeviceProp_t prop;
-    size_t lds = 65536;
-    if (hipGetDeviceProperties(&prop, gpu_device < 0 ? 0 : gpu_device) == hipSuccess) lds = prop.sharedMemPerBlock;
-    const int N = L * L;
-    if (lds_bytes_for(2, N, 1, 1, 64) <= lds) return 2;
-    if ((size_t)kLdsLattice + (size_t)layout_stride(1, N) * 4 <= lds) return 1;
-    return 0;
+    size_t shmem = 65536;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) shmem = prop.sharedMemPerBlock;
+    return select_method_for(L, (long long)shmem, threadsPerBlock);
 }
 
 // ============================================================ driver

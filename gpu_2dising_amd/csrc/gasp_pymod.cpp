@@ -17,7 +17,9 @@
 // gridinput.bin instead of dereferencing a missing list; the GIL is released
 // while the GPU runs and re-taken for output; keyword `seed` (default: the
 // clock, as the reference) and `rng` ('philox' or 'mt19937') are added;
-// without a GPU the run functions raise instead of falling back to a CPU path;
+// without a GPU (import prints "Falling back to CPU", as the reference) the
+// run functions use the reference's CPU engine, mc_driver_cpu with one MT19937
+// stream seeded with `seed`, and `rng` / `gpu_method` do not apply;
 // keyword `write_hdf5` (default True, as the reference always writes) can skip
 // the HDF5 grid file (gridstates.hdf5 / pBgrids.hdf5, written through
 // libgasp_hdf5 in the reference's layout).
@@ -296,14 +298,6 @@ extern "C" int append_grids_list(int L, int ngrids, int *grid_data, int isweep, 
 }
 
 // =================================================================== helpers
-static bool require_gpu() {
-    if (g_idev < 0) {
-        PyErr_SetString(PyExc_RuntimeError, "gasp: no GPU visible to the engine (this build has no CPU fallback)");
-        return false;
-    }
-    return true;
-}
-
 static bool parse_seed(PyObject *seed_obj, unsigned long *seed) {
     if (!seed_obj || seed_obj == Py_None) {
         *seed = (unsigned long)time(nullptr);
@@ -316,6 +310,10 @@ static bool parse_seed(PyObject *seed_obj, unsigned long *seed) {
 }
 
 static bool parse_method(PyObject *obj, int L, int ngrids, int tpb, int *method) {
+    if (g_idev < 0) {  // the CPU engine has no methods
+        *method = 0;
+        return true;
+    }
     if (!obj || obj == Py_None) {
         *method = select_gpu_method(L, ngrids, tpb, g_idev);
         return true;
@@ -332,10 +330,56 @@ static PyObject *raise_lib(const char *what) {
 }
 
 // Runs the driver with the GIL released; returns false with a Python error set.
+// Without a GPU (gpuInitDevice returned -1 at import) the reference's CPU
+// engine runs instead: one MT19937 stream seeded with `seed`
+// (src/ising_ext.cu:637-664, 1007-1030); `rng`, `tpb` and `method` do not apply.
+static bool run_driver_cpu(int L, int ngrids, int *ising_grids, int *grid_fate, double beta, double h,
+                           int tot_nsweeps, int mag_output_int, int grid_output_int, int itask, int initial_spin,
+                           const char *cv, double dn, double up, int ninputs, float *result, unsigned long seed,
+                           const char *filename) {
+    std::string cvs(cv), fname(filename);
+    mc_grids_t grids{L, ngrids, ising_grids};
+    mc_sampler_t samples{tot_nsweeps, mag_output_int, grid_output_int};
+    mc_function_t calc;
+    calc.itask = itask;
+    calc.dn_thr = dn;
+    calc.up_thr = up;
+    calc.ninputs = ninputs;
+    calc.initial_spin = initial_spin;
+    calc.cv = &cvs[0];
+    calc.result = result;
+    calc.filename = &fname[0];
+    g_run.failed = false;
+    g_run.err_type = g_run.err_value = g_run.err_tb = nullptr;
+    init_genrand(seed);
+    preComputeProbs_cpu(beta, h);
+    if (g_run.write_hdf5) create_ising_grids_hdf5(L, ngrids, tot_nsweeps, h, beta, itask, &fname[0]);
+    gasp_set_output(py_print, 1);
+    int rc;
+    Py_BEGIN_ALLOW_THREADS
+    mc_driver_cpu(grids, beta, h, grid_fate, samples, calc, append_grids_list);
+    rc = gasp_last_error();
+    Py_END_ALLOW_THREADS
+    gasp_set_output(nullptr, 0);
+    if (g_run.failed) {
+        PyErr_Restore(g_run.err_type, g_run.err_value, g_run.err_tb);
+        g_run.err_type = g_run.err_value = g_run.err_tb = nullptr;
+        return false;
+    }
+    if (rc) {
+        raise_lib("mc_driver_cpu");
+        return false;
+    }
+    return true;
+}
+
 static bool run_driver(int L, int ngrids, int *ising_grids, int *grid_fate, double beta, double h, int tot_nsweeps,
                        int mag_output_int, int grid_output_int, int itask, int initial_spin, const char *cv,
                        double dn, double up, int ninputs, float *result, int tpb, int method, unsigned long seed,
                        const char *rng, const char *filename) {
+    if (g_idev < 0)
+        return run_driver_cpu(L, ngrids, ising_grids, grid_fate, beta, h, tot_nsweeps, mag_output_int,
+                              grid_output_int, itask, initial_spin, cv, dn, up, ninputs, result, seed, filename);
     int *d_grids = nullptr, *d_nb = nullptr;
     gpuInitGrid(L, ngrids, tpb, ising_grids, &d_grids, &d_nb);
     if (gasp_last_error()) {
@@ -472,7 +516,6 @@ static PyObject *run_nucleation_swarm(PyObject *, PyObject *args, PyObject *kwar
     unsigned long seed;
     if (!parse_seed(seed_obj, &seed)) return nullptr;
     int method;
-    if (!require_gpu()) return nullptr;
     if (!parse_method(method_obj, L, ngrids, tpb, &method)) return nullptr;
     if (!begin_history(L, ngrids, tot_nsweeps, grid_output_int, keep_grids, max_keep_grids)) return nullptr;
 
@@ -610,7 +653,6 @@ static PyObject *run_committor_calc(PyObject *, PyObject *args, PyObject *kwargs
     }
     unsigned long seed;
     if (!parse_seed(seed_obj, &seed)) return nullptr;
-    if (!require_gpu()) return nullptr;
     int method;
     if (!parse_method(method_obj, L, ngrids, tpb, &method)) return nullptr;
 
@@ -714,6 +756,8 @@ PyMODINIT_FUNC PyInit_gasp(void) {
     // src/ising_ext.cu:1179-1189: initialise the device at import
     g_idev = gpuInitDevice(-1, &g_gpu_nsms);
     if (g_idev < 0) {
+        printf("Falling back to CPU\n");
+        fflush(stdout);
         g_gpu_nsms = 0;
         gasp_clear_error();
     }

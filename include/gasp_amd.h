@@ -15,8 +15,10 @@
  *  - Errors never call exit(): the function prints the reference's message to
  *    stderr, records an error code readable with gasp_last_error(), and
  *    returns.
- *  - There is no CPU fallback inside the library (gpuInitDevice returns -1
- *    when no GPU is visible and the caller decides what to do).
+ *  - gpuInitDevice returns -1 when no GPU is visible; the caller then falls
+ *    back to the CPU engine declared below (mc_driver_cpu and its helpers,
+ *    the reference's single global MT19937 stream), as the reference's CLI
+ *    and Python module do.
  *
  * All device pointers returned by this library are hipMalloc allocations and
  * may be released with hipFree() or gasp_free().
@@ -111,9 +113,11 @@ void gpuInitRand(int ngrids, int threadsPerBlock, unsigned long rngseed,
  * Process-global, like the reference's __constant__ table. */
 void preComputeProbs_gpu(double beta, double h);
 
-/* include/gpu_tools.h:38 (src/gpu_tools.cu:205-232).  2 when a 64-replica
- * multispin lattice (8*L*L bytes) fits in one workgroup's LDS, else 1 when a
- * single replica's bit lattice fits, else 0. */
+/* include/gpu_tools.h:38 (src/gpu_tools.cu:205-232).  The reference's rule:
+ * 2 when L <= maxL = floor(sqrt(32 floor(m^2 / 32))), m = floor(sqrt(8 shmem /
+ * threadsPerBlock)), shmem = the device's shared memory per block (64 KB is
+ * assumed when no device answers), else 0.  gpu_device -1 means the current
+ * device. */
 int select_gpu_method(int L, int ngrids, int threadsPerBlock, int gpu_device);
 
 /* include/mc_gpu.h:47 (src/mc_gpu.cu:530-831).  Same schedule, outputs and
@@ -144,6 +148,39 @@ int write_ising_grids(int L, int ngrids, int *ising_grids, int isweep,
 /* include/bootstrap.h:6 (src/bootstrap.c:6-63).  m resamples drawn with libc
  * rand(); returns the population standard deviation of the resampled pB. */
 double bootstrap_errbar(int ngrids, int m, int *grid_fate_host);
+
+/* ------------------------------------------------------------ CPU fallback */
+/* The reference's CPU engine (include/mc_cpu.h, include/mt19937ar.h), used by
+ * the callers when gpuInitDevice returns -1.  One process-global MT19937
+ * stream (not thread-safe, as the reference's). */
+
+/* include/mt19937ar.h:47 (src/mt19937ar.c:60-73): seeds the global stream. */
+void init_genrand(unsigned long s);
+/* include/mt19937ar.h:56 (src/mt19937ar.c:105-140): next 32-bit output. */
+unsigned long genrand_int32(void);
+/* include/mt19937ar.h:69 (src/mt19937ar.c:163-167): (u + 0.5) / 2^32. */
+double genrand_real3(void);
+/* include/mc_cpu.h:34 (src/mc_cpu.c:14-38): process-global double table. */
+void preComputeProbs_cpu(double beta, double h);
+/* include/mc_cpu.h:37 (src/mc_cpu.c:41-89): nsweeps*L*L random-site
+ * Metropolis attempts on grid grid_index, three draws per attempt (row,
+ * column, acceptance) from the global stream; beta and h are unused (the
+ * table holds them). */
+void mc_sweep_cpu(int L, int *ising_grids, int grid_index, double beta, double h, int nsweeps);
+/* include/mc_cpu.h:40 (src/mc_cpu.c:91-104): magnetisation[grid_index] =
+ * (float)(sum / (double)(L*L)). */
+void compute_magnetisation_cpu(int L, int *ising_grids, int grid_index, float *magnetisation);
+/* include/mc_cpu.h:43 (src/mc_cpu.c:310-364): size of the largest
+ * 4-connected periodic cluster of `spin` in grid grid_index, as a float. */
+void compute_largest_cluster_cpu(int L, int *ising_grids, const int grid_index, int spin, float *lclus_size);
+/* include/mc_cpu.h:46 (src/mc_cpu.c:106-308).  The reference's CPU schedule:
+ * at each check the collective variable of every replica, the report (same
+ * stdout text) and the stopping test, then outfunc at grid_output_int, then
+ * min(mag, grid) sweeps of each replica in turn from the one global stream
+ * (seed it with init_genrand, and call preComputeProbs_cpu, first).  Same
+ * outputs as mc_driver_gpu.  Errors set gasp_last_error() and return. */
+void mc_driver_cpu(mc_grids_t grids, double beta, double h, int *grid_fate, mc_sampler_t samples,
+                   mc_function_t calc, GridOutputFunc outfunc);
 
 /* ================================================== engine-specific additions */
 /* No counterpart in the reference. */
@@ -260,8 +297,6 @@ int gasp_graph_create(gasp_engine **engines, int n, const int *nsweeps, const in
 int gasp_graph_launch(gasp_graph *g, int replays); /* runs and waits */
 double gasp_graph_ms(gasp_graph *g);               /* GPU time of all replays */
 int gasp_graph_destroy(gasp_graph *g);
-/* Times the candidate launch shapes (slots per replica, waves per workgroup)
- * for (method, L, ngrids) and keeps the fastest for later engines. */
 /* bootstrap_errbar (src/bootstrap.c:6-63) for `ninputs` inputs of n fates each
  * (int32 [ninputs][n], -1/0/1; host memory, or device memory if on_device)
  * on the GPU: m resamples per input, the same estimator with Philox4x32-10
@@ -280,8 +315,15 @@ int gasp_bootstrap_errbar_gpu(const int *fates, int on_device, int ninputs, int 
  * own rand() state.  out[ninputs]. */
 int gasp_bootstrap_errbar_libc(const int *fates, int ninputs, int n, int m, unsigned seed, int nthreads,
                                double *out);
-int gasp_autotune(int method, int L, int ngrids, int probe_sweeps, int *best_slots, int *best_waves,
-                  double *best_rate);
+/* Times the candidate launch shapes (slots per replica x waves per workgroup,
+ * or word split for method 2) for (method, L, ngrids) on probe_sweeps sweeps
+ * at (beta, h) from all spins down, and keeps the fastest for later engines
+ * of that (method, L, ngrids).  With check_sweeps > 0 it then times that
+ * interval as 1, 2, 4, ... equal launches (of at least check_sweeps / 8
+ * sweeps) and returns the fastest sweeps per launch in *best_launch_sweeps
+ * (check_sweeps when 0 or 1).  *best_rate: attempts/s of the chosen shape. */
+int gasp_autotune(int method, int L, int ngrids, double beta, double h, int probe_sweeps, int check_sweeps,
+                  int *best_slots, int *best_waves, int *best_launch_sweeps, double *best_rate);
 
 /* ------------------------------------------------ committor surrogate (CNN) */
 /* No counterpart in the reference: the learned addition.  A 4-layer circular

@@ -195,3 +195,68 @@ def driver(L, R, grids, *, rng_kind, seed, beta, h, tot_nsweeps, mag_output_int,
         "dumps": dumps[:nd],
         "dump_sweeps": dsw[:nd],
     }
+
+
+def gpu_thresholds(beta: float, h: float) -> np.ndarray:
+    """Integer thresholds (20 entries, reference index 5(s+1)+nsum+4) of the
+    single-precision table src/mc_gpu.cu:21-37."""
+    return thresholds(pacc_gpu(beta, h))
+
+
+def sweep_philox_replicas(L: int, grids: np.ndarray, replicas, seed: int, attempt0: int, nattempts: int,
+                          beta: float, h: float, nthreads: int = 8) -> np.ndarray:
+    """Production-stream sweeps of selected replicas: grids[k] (int32 [L, L])
+    is advanced as replica replicas[k] of an engine keyed by `seed`, from
+    attempt index attempt0, in place (threads over the replicas)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    thr = np.ascontiguousarray(gpu_thresholds(beta, h), dtype=np.uint64)
+    flat = np.ascontiguousarray(grids.reshape(len(replicas), L * L), dtype=np.int32)
+    rows = [np.ascontiguousarray(flat[k]) for k in range(len(replicas))]
+
+    def one(k):
+        lib().orc_sweep_philox(L, rows[k], seed, int(replicas[k]), attempt0, nattempts, thr)
+
+    with ThreadPoolExecutor(max_workers=max(1, nthreads)) as ex:
+        list(ex.map(one, range(len(replicas))))
+    out = np.stack(rows).reshape(grids.shape)
+    grids[...] = out
+    return grids
+
+
+def magnetisation_gpu(grids: np.ndarray) -> np.ndarray:
+    """(float)sum / (float)N per grid (src/mc_gpu.cu:445-449), grids [R, L, L]."""
+    R = grids.shape[0]
+    s = grids.reshape(R, -1).sum(1)
+    return (s.astype(np.float32) / np.float32(grids[0].size)).astype(np.float32)
+
+
+def shoot_replicas(L: int, grids: np.ndarray, replicas, seed: int, beta: float, h: float, chk: int,
+                   dn: float, up: float, max_sweeps: int, nthreads: int = 8):
+    """Committor shooting of selected replicas with decided replicas frozen:
+    check the start state, then sweep chk sweeps at a time (attempt indices
+    from 0, one counter for the whole engine) and check again, until the fate
+    is decided (1: m > up, 0: m < dn, sticky) or max_sweeps.  Returns
+    (fates, grids advanced in place, sweeps swept per replica)."""
+    n = len(replicas)
+    fates = np.full(n, -1, dtype=np.int32)
+    swept = np.zeros(n, dtype=np.int64)
+    N = L * L
+
+    def decide(idx):
+        m = magnetisation_gpu(grids[idx]).astype(np.float64)
+        fates[idx] = np.where(m > up, 1, np.where(m < dn, 0, -1))
+
+    decide(np.arange(n))
+    done = 0
+    while done < max_sweeps:
+        act = np.flatnonzero(fates == -1)
+        if act.size == 0:
+            break
+        g = grids[act].copy()
+        sweep_philox_replicas(L, g, [replicas[i] for i in act], seed, done * N, chk * N, beta, h, nthreads)
+        grids[act] = g
+        swept[act] += chk
+        done += chk
+        decide(act)
+    return fates, grids, swept

@@ -6,6 +6,7 @@ Observable outputs compared exactly: the result array (nucleated fraction or
 pB), the fates, and everything handed to the output callback (sweep number,
 host grids, magnetisation / cluster sizes)."""
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -130,13 +131,31 @@ def test_config1_against_reference_goldens(gpu, engine_lib, oracle, method):
     assert np.array_equal(out["result"], frac)
 
 
+def _max_l(shmem: int, tpb: int) -> int:
+    """src/gpu_tools.cu:212-214 evaluated step by step."""
+    m = int(math.sqrt(8.0 * shmem / tpb))
+    return int(math.sqrt(32 * (m * m // 32)))
+
+
 def test_select_gpu_method(gpu, engine_lib):
+    """The reference's rule (src/gpu_tools.cu:205-232): 2 up to maxL, 0 beyond, at the
+    device's shared memory per block (160 KB on MI355X: maxL 201 at 32 threads per
+    block, 142 at 64)."""
     lib = engine_lib.lib()
-    assert lib.select_gpu_method(64, 1024, 32, 0) == 2
-    assert lib.select_gpu_method(32, 1024, 64, 0) == 2
+    hip = C.CDLL("libamdhip64.so")
+    v = C.c_int()
+    # hipDeviceAttributeMaxSharedMemoryPerBlock
+    attr = 74
+    assert hip.hipDeviceGetAttribute(C.byref(v), attr, 0) == 0
+    shmem = v.value
+    assert shmem == 160 * 1024
+    assert (_max_l(shmem, 32), _max_l(shmem, 64)) == (201, 142)
+    for tpb, edge in ((32, 201), (64, 142), (128, _max_l(shmem, 128))):
+        assert lib.select_gpu_method(edge, 1024, tpb, 0) == 2
+        assert lib.select_gpu_method(edge + 1, 1024, tpb, 0) == 0
+        assert lib.select_gpu_method(edge, 1024, tpb, -1) == 2  # current device
     assert lib.select_gpu_method(4000, 16, 32, 0) == 0
-    m = lib.select_gpu_method(400, 64, 32, 0)
-    assert m in (0, 1)
+    assert lib.select_gpu_method(2, 16, 64, 0) == 2
 
 
 def test_driver_argument_errors(gpu, engine_lib):

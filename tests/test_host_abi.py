@@ -2,7 +2,8 @@
 function include/gasp_amd.h declares; the host-side pieces (initial grids,
 fates, gridinput.bin / gridstates.bin, bootstrap error bar) match the
 reference's formats and golden values; the CLI and the `gasp` extension
-behave like the reference's when no GPU is visible (minus the CPU fallback)."""
+behave like the reference's when no GPU is visible (the CPU fallback itself is
+tests/test_cpu_fallback.py)."""
 import ctypes as C
 import os
 import re
@@ -173,13 +174,13 @@ def test_cli_without_gpu(engine_lib, tmp_path):
     assert os.path.exists(cli)
     r = subprocess.run([cli], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode != 0 and "Usage : GPU_2DIsing nsweeps nreplicas" in r.stdout
+    # without gridinput.bin an itask-1 run stops before any engine runs; the
+    # old gridstates.bin is deleted at start all the same (src/ising.cu:84)
     (tmp_path / "gridstates.bin").write_bytes(b"old")
-    r = subprocess.run([cli, "100", "64", "10", "10", "32", "0", "2", "0.54", "0.07", "0"],
+    r = subprocess.run([cli, "100", "64", "10", "10", "32", "0", "2", "0.54", "0.07", "1"],
                        capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode != 0
-    assert r.stdout.startswith("# isweep    nucleated fraction")
-    assert "no CPU fallback" in r.stderr
-    assert not (tmp_path / "gridstates.bin").exists()  # deleted at start (src/ising.cu:84)
+    assert not (tmp_path / "gridstates.bin").exists()
 
 
 @pytest.mark.skipif(_gpu_visible(), reason="checks the no-GPU behaviour")
@@ -207,10 +208,6 @@ try:
     gasp.run_nucleation_swarm(64, 128, 100, 0.54, 0.07, max_keep_grids=-1); raise SystemExit("bad max")
 except ValueError:
     pass
-try:
-    gasp.run_nucleation_swarm(64, 128, 100, 0.54, 0.07); raise SystemExit("ran without GPU")
-except RuntimeError as e:
-    assert "no CPU fallback" in str(e)
 try:
     gasp.run_committor_calc(8, 64, 100, 0.54, 0.07, nsms=-1); raise SystemExit("bad nsms")
 except ValueError:
