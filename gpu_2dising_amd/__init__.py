@@ -35,7 +35,7 @@ EXPORTS = (
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
-    "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream",
+    "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream", "gasp_sweep_grids",
     "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
     "gasp_bootstrap_errbar_gpu", "gasp_bootstrap_errbar_libc",
     "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
@@ -156,6 +156,8 @@ def lib() -> C.CDLL:
     L.gasp_engine_kernel_ms.restype = C.c_double
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
+    L.gasp_sweep_grids.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
+    L.gasp_sweep_grids.restype = C.c_int
     dp = C.POINTER(C.c_double)
     L.gasp_graph_create.argtypes = [C.POINTER(vp), C.c_int, ip, ip, ip, dp, dp, C.POINTER(vp)]
     L.gasp_graph_create.restype = C.c_int

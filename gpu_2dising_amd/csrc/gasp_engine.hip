@@ -158,22 +158,10 @@ int select_method_for(int L, long long shmem, int tpb) {
 }  // namespace
 
 // ============================================================ random streams
-// gasp_rng_state: a device allocation whose first 64 bytes are this header;
-// per-replica MT19937 states (interleaved [624][ngrids]) follow at offset 256.
-struct RngHeader {
-    uint32_t magic;
-    uint32_t kind;  // 0 = Philox4x32-10, 1 = MT19937 per replica
-    uint64_t seed;
-    int32_t ngrids;
-    int32_t reserved0;
-    uint64_t attempts;  // attempts consumed per replica
-    uint64_t reserved[4];
-};
-static_assert(sizeof(RngHeader) == 64, "header size");
-static constexpr uint32_t kRngMagic = 0x474E5247u;  // "GRNG"
+// gasp_rng_state: a device allocation whose first 64 bytes are an RngHeader
+// (gasp_kernels.h); per-replica MT19937 states (interleaved [624][ngrids])
+// follow at offset 256.
 static constexpr size_t kRngMtOffset = 256;
-
-enum { KIND_PHILOX = 0, KIND_MT = 1 };
 
 static int rng_alloc(int ngrids, uint64_t seed, int kind, gasp_rng_state **out) {
     if (!out) return set_error(GASP_ERR_ARG, "gpuInitRand: NULL output pointer");
@@ -422,6 +410,45 @@ static int ilog2(int L) {
     return g;
 }
 
+// ============================================================ launch shape
+// Slots per replica, waves per workgroup (word split for method 2), the
+// native layout's stride and size, and the workgroup geometry of the sweep
+// kernel for (method, L, R).
+struct LaunchShape {
+    int S = 1, W = 1, stride = 0, rpw = 0, nblocks = 0, block = 0;
+    int tbl = 1;  // sweep_hbm_waves: conflict-table entries
+    int ncu = 256;
+    size_t lds = 0, lat_bytes = 0;
+};
+
+static int launch_shape(int method, int L, int R, LaunchShape *sh) {
+    const int N = L * L;
+    choose_tuning(method, L, R, &sh->S, &sh->W);
+    sh->stride = layout_stride(method, N);
+    DeviceInfo di = device_info();
+    if (di.ok) sh->ncu = di.ncu;
+    sh->lds = lds_bytes_for(method, N, sh->S, sh->W, R);
+    sh->tbl = (method == 0 && sh->S > 64) ? hbm_table_entries(N, R) : 1;
+    if (method == 2) {
+        const int groups = (R + 63) / 64;
+        sh->rpw = 64 / sh->W;
+        sh->block = sh->rpw * sh->S;
+        sh->nblocks = groups * (64 / sh->rpw);
+        sh->lat_bytes = (size_t)groups * sh->stride * 8;
+    } else {
+        sh->block = 64 * sh->W;
+        sh->rpw = std::max(1, sh->W * 64 / sh->S);
+        sh->nblocks = (R + sh->rpw - 1) / sh->rpw;
+        sh->lat_bytes = method == 1 ? (size_t)R * sh->stride * 4 : (size_t)R * sh->stride;
+    }
+    if (di.ok && sh->lds > di.lds_block)
+        return set_error(GASP_ERR_ARG,
+                         "Error: gpu_method %d needs %zu bytes of LDS per workgroup at L=%d (device "
+                         "limit %zu); use select_gpu_method",
+                         method, sh->lds, L, (size_t)di.lds_block);
+    return GASP_OK;
+}
+
 // ============================================================ engine object
 struct gasp_engine {
     int L = 0, N = 0, R = 0, method = 0;
@@ -514,34 +541,23 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
         memcpy(e->thr, thr, sizeof e->thr);
     }
 
-    choose_tuning(method, L, R, &e->S, &e->W);
+    LaunchShape sh;
+    if (launch_shape(method, L, R, &sh)) {
+        engine_free(e);
+        return gasp_last_error();
+    }
+    e->S = sh.S;
+    e->W = sh.W;
     e->S0 = e->S;
     e->adapt = method == 2 && g_tune_slots <= 0;
-    e->stride = layout_stride(method, e->N);
-    DeviceInfo di = device_info();
-    if (di.ok) e->ncu = di.ncu;
-    e->lds = lds_bytes_for(method, e->N, e->S, e->W, R);
-    e->tbl = (method == 0 && e->S > 64) ? hbm_table_entries(e->N, R) : 1;
-    if (method == 2) {
-        const int groups = (R + 63) / 64;
-        e->rpw = 64 / e->W;
-        e->block = e->rpw * e->S;
-        e->nblocks = groups * (64 / e->rpw);
-        e->lat_bytes = (size_t)groups * e->stride * 8;
-    } else {
-        e->block = 64 * e->W;
-        e->rpw = std::max(1, e->W * 64 / e->S);
-        e->nblocks = (R + e->rpw - 1) / e->rpw;
-        e->lat_bytes = method == 1 ? (size_t)R * e->stride * 4 : (size_t)R * e->stride;
-    }
-    if (di.ok && e->lds > di.lds_block) {
-        const size_t need = e->lds;
-        engine_free(e);
-        return set_error(GASP_ERR_ARG,
-                         "Error: gpu_method %d needs %zu bytes of LDS per workgroup at L=%d (device "
-                         "limit %zu); use select_gpu_method",
-                         method, need, L, (size_t)di.lds_block);
-    }
+    e->stride = sh.stride;
+    e->ncu = sh.ncu;
+    e->lds = sh.lds;
+    e->tbl = sh.tbl;
+    e->rpw = sh.rpw;
+    e->block = sh.block;
+    e->nblocks = sh.nblocks;
+    e->lat_bytes = sh.lat_bytes;
     SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L));
     if (!fn) {
         engine_free(e);
@@ -645,7 +661,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.a_begin = e->rng.attempts;
     // in a captured graph the attempt counter lives in the RNG header on the
     // device and is advanced by a kernel after the sweep
-    a.a_begin_dev = capture ? (const uint64_t *)((char *)e->d_rng + offsetof(RngHeader, attempts)) : nullptr;
+    a.rng_dev = capture ? (const RngHeader *)e->d_rng : nullptr;
     a.n_attempts = n_attempts;
     a.seed = e->rng.seed;
     a.draws = draws;
@@ -666,8 +682,8 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     if (capture) {
         hipLaunchKernelGGL(fn, dim3(e->nblocks), dim3(e->block), e->lds, e->stream, a);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(advance_counter, dim3(1), dim3(64), 0, e->stream,
-                           (uint64_t *)((char *)e->d_rng + offsetof(RngHeader, attempts)), n_attempts);
+        hipLaunchKernelGGL(advance_counter, dim3(1), dim3(64), 0, e->stream, (RngHeader *)e->d_rng, e->R,
+                           n_attempts);
         HIP_TRY(hipGetLastError());
         return GASP_OK;
     }
@@ -1034,6 +1050,130 @@ extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
 }
 
 extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+// ============================================================ caller grids
+// Stream-ordered sweeps of caller-owned int32 device grids (the layout
+// gpuInitGrid makes): the granularity of the reference driver's per-interval
+// launches (src/mc_gpu.cu:675-708: mc_sweep_gpu*, then
+// compute_magnetisation_gpu) without an engine object.  The Philox key and
+// the attempt counter come from the RNG header in device memory and the
+// counter is advanced there by a kernel, so nothing is read back to the host
+// and consecutive calls continue one chain per replica.  A workspace holds,
+// at 256-byte aligned offsets: the 10 thresholds (0), 4 counters (128), the
+// magnetisation when the caller passes none (256: R floats) and the native
+// lattice.
+
+// Stores the thresholds and clears the counters (a kernel rather than a host
+// copy or a memset, so that the sequence can be captured and replayed).
+struct Thresholds10 {
+    uint64_t v[10];
+};
+__global__ void set_thresholds(Thresholds10 t, uint64_t *thr, int *counts) {
+    const int i = threadIdx.x;
+    if (i < 10) thr[i] = t.v[i];
+    if (i < 4) counts[i] = 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t grids_ws_lattice_offset(int R, bool own_mag) {
+    return own_mag ? align256(256 + (size_t)R * sizeof(float)) : 256;
+}
+
+static int grids_check_args(int L, int R, int method, gasp_rng_state *d_state, const int *d_grids, int nsweeps) {
+    if (L < 2 || L > 8192) return set_error(GASP_ERR_ARG, "Error: L must lie in [2, 8192]");
+    if (R < 1) return set_error(GASP_ERR_ARG, "Error: ngrids must be positive");
+    if (method < 0 || method > 2) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
+    if (!d_state) return set_error(GASP_ERR_ARG, "Error: RNG state is NULL (call gpuInitRand)");
+    if (!d_grids) return set_error(GASP_ERR_ARG, "Error: device grids are NULL");
+    if (nsweeps < 0) return set_error(GASP_ERR_ARG, "Error: nsweeps must not be negative");
+    return GASP_OK;
+}
+
+// The workspace must hold grids_ws_lattice_offset(R, !d_mag) + sh.lat_bytes.
+static int sweep_grids_run(int L, int R, int method, gasp_rng_state *d_state, double beta, double h, int *d_grids,
+                           int nsweeps, float *d_mag, hipStream_t st, const LaunchShape &sh, unsigned char *ws) {
+    const int N = L * L;
+    float pf[20];
+    Thresholds10 t;
+    table_f32(beta, h, pf);
+    thr_from_table_f(pf, t.v);
+    uint64_t *d_thr = (uint64_t *)ws;
+    int *d_counts = (int *)(ws + 128);
+    float *mag = d_mag ? d_mag : (float *)(ws + 256);
+    void *lat = ws + grids_ws_lattice_offset(R, !d_mag);
+    hipLaunchKernelGGL(set_thresholds, dim3(1), dim3(64), 0, st, t, d_thr, d_counts);
+    HIP_TRY(hipGetLastError());
+    const dim3 grid(1024), blk(256);
+    if (method == 2)
+        hipLaunchKernelGGL(pack_multispin, grid, blk, 0, st, d_grids, (unsigned *)lat, R, N, sh.stride, 1);
+    else if (method == 1)
+        hipLaunchKernelGGL(pack_bits, grid, blk, 0, st, d_grids, (unsigned *)lat, R, N, sh.stride, 1);
+    else
+        hipLaunchKernelGGL(pack_hbm, grid, blk, 0, st, d_grids, (unsigned char *)lat, R, N, sh.stride, 1);
+    HIP_TRY(hipGetLastError());
+    SweepFn fn = sweep_fn(method, 0, sh.S, sh.W, is_pow2(L));
+    if (!fn) return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", sh.S);
+    if (sh.lds > 65536)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
+    SweepArgs a{};
+    a.lattice = lat;
+    a.L = L;
+    a.N = N;
+    a.R = R;
+    a.lg = ilog2(L);
+    a.rpw = sh.rpw;
+    a.stride = sh.stride;
+    a.rng_dev = (const RngHeader *)d_state;
+    a.thr = d_thr;
+    a.mag = mag;
+    a.counts = d_counts;
+    a.check = CHECK_NONE;
+    a.tbl_mask = (unsigned)(sh.tbl - 1);
+    // launches of at most 2^30 attempts per replica (32-bit launch-local
+    // counters); with no sweeps one launch still computes the magnetisation
+    uint64_t left = (uint64_t)nsweeps * (uint64_t)N;
+    do {
+        const uint64_t nb = std::min<uint64_t>(left, 1ull << 30);
+        a.n_attempts = nb;
+        hipLaunchKernelGGL(fn, dim3(sh.nblocks), dim3(sh.block), sh.lds, st, a);
+        HIP_TRY(hipGetLastError());
+        if (nb) {
+            hipLaunchKernelGGL(advance_counter, dim3(1), dim3(64), 0, st, (RngHeader *)d_state, R, nb);
+            HIP_TRY(hipGetLastError());
+        }
+        left -= nb;
+    } while (left > 0);
+    if (method == 2)
+        hipLaunchKernelGGL(unpack_multispin<int>, grid, blk, 0, st, (const unsigned *)lat, d_grids, R, N, sh.stride);
+    else if (method == 1)
+        hipLaunchKernelGGL(unpack_bits<int>, grid, blk, 0, st, (const unsigned *)lat, d_grids, R, N, sh.stride);
+    else
+        hipLaunchKernelGGL(unpack_hbm<int>, grid, blk, 0, st, (const unsigned char *)lat, d_grids, R, N, sh.stride);
+    HIP_TRY(hipGetLastError());
+    return GASP_OK;
+}
+
+extern "C" int gasp_sweep_grids(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta, double h,
+                                int *d_ising_grids, int nsweeps, float *d_mag, void *stream) {
+    clear_error();
+    int rc = grids_check_args(L, ngrids, gpu_method, d_state, d_ising_grids, nsweeps);
+    if (rc) return rc;
+    LaunchShape sh;
+    if ((rc = launch_shape(gpu_method, L, ngrids, &sh))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = grids_ws_lattice_offset(ngrids, !d_mag) + sh.lat_bytes;
+    unsigned char *ws = nullptr;
+    hipError_t he = hipMallocAsync((void **)&ws, bytes, st);
+    if (he != hipSuccess)
+        return set_error(he == hipErrorOutOfMemory ? GASP_ERR_NOMEM : GASP_ERR_HIP, "GPUassert: %s %s %d",
+                         hipGetErrorString(he), __FILE__, __LINE__);
+    rc = sweep_grids_run(L, ngrids, gpu_method, d_state, beta, h, d_ising_grids, nsweeps, d_mag, st, sh, ws);
+    he = hipFreeAsync(ws, st);
+    if (!rc && he != hipSuccess)
+        rc = set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(he), __FILE__, __LINE__);
+    return rc;
+}
 
 // ============================================================ GPU bootstrap
 // bootstrap_errbar (src/bootstrap.c:6-63) for many inputs at once, with

@@ -68,6 +68,26 @@ __device__ __forceinline__ unsigned lds_origin(unsigned char *smem) {
     return (unsigned)(uintptr_t)(lds_u8 *)smem;
 }
 
+// Header of a gasp_rng_state device allocation (its first 64 bytes).
+struct RngHeader {
+    uint32_t magic;
+    uint32_t kind;  // KIND_PHILOX or KIND_MT
+    uint64_t seed;
+    int32_t ngrids;
+    int32_t reserved0;
+    uint64_t attempts;  // attempts consumed per replica
+    uint64_t reserved[4];
+};
+static_assert(sizeof(RngHeader) == 64, "header size");
+constexpr uint32_t kRngMagic = 0x474E5247u;  // "GRNG"
+enum { KIND_PHILOX = 0, KIND_MT = 1 };
+
+// A header the sweep kernels may take their Philox key and attempt counter
+// from: made by gpuInitRand, counter-based, with streams for R replicas.
+__device__ __forceinline__ bool rng_header_ok(const RngHeader *h, int R) {
+    return h->magic == kRngMagic && h->kind == (uint32_t)KIND_PHILOX && h->ngrids >= R;
+}
+
 struct SweepArgs {
     void *lattice;           // native layout (see engine)
     int L, N, R;
@@ -75,7 +95,8 @@ struct SweepArgs {
     int rpw;                 // replicas per workgroup (layouts 0/1)
     int stride;              // layout 0: bytes per replica; 1: dwords per replica; 2: dwords per plane (2 per group)
     uint64_t a_begin;        // per-replica attempt index of the first attempt
-    const uint64_t *a_begin_dev;  // if set, a_begin is read from device memory (graph replays)
+    const RngHeader *rng_dev;  // if set, a_begin and seed are read from this device header
+                               // (graph replays, stream-ordered sweeps of caller grids)
     uint64_t n_attempts;     // attempts per replica in this launch (< 2^31)
     uint64_t seed;           // Philox key
     const uint32_t *draws;   // RNG_DRAWS: [R][3*n_attempts], draw stream from a_begin
@@ -499,9 +520,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     // (replicas past R, and when freezing those with a decided fate, do not)
     const bool live = r < a.R;
     const bool active = live && !(a.freeze && a.fate[r] != -1);
-    // absolute LDS addresses assume smem at address 0: otherwise leave the
-    // lattice alone and report NaN magnetisations
-    if (lds_origin(smem) != 0u) {
+    // absolute LDS addresses assume smem at address 0, and a device header
+    // must be a Philox one for these replicas: otherwise leave the lattice
+    // alone and report NaN magnetisations (uniform over the workgroup)
+    if (lds_origin(smem) != 0u || (a.rng_dev && !rng_header_ok(a.rng_dev, a.R))) {
         if (live && slot == 0) a.mag[r] = __builtin_nanf("");
         return;
     }
@@ -560,12 +582,13 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 
     // Launch-local attempt coordinates shifted by b = a_begin & 1 so that
     // attempt pairs (2q, 2q+1) coincide with global Philox pairs P0 + q.
-    const uint64_t a_begin = a.a_begin_dev ? *a.a_begin_dev : a.a_begin;
+    const uint64_t a_begin = a.rng_dev ? a.rng_dev->attempts : a.a_begin;
+    const uint64_t seed = a.rng_dev ? a.rng_dev->seed : a.seed;
     const uint32_t b = (uint32_t)(a_begin & 1ull);
     const uint64_t P0 = a_begin >> 1;
     const uint32_t e = (uint32_t)a.n_attempts + b;
     uint32_t l0 = active ? b : e;
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     // stencil positions: byte offsets in the split layouts (entries of 4, 2, 1 bytes)
     constexpr int U = (LAT == LAT_MULTIHALF) ? 4 : (LAT == LAT_MULTI16) ? 2 : 1;
     const Geometry<POW2, U> geo{L, N, a.lg};
@@ -844,7 +867,7 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     const int r = blockIdx.x;  // grid = R workgroups
     // uniform over the workgroup
     const bool active = !(a.freeze && a.fate[r] != -1);
-    if (lds_origin(smem) != 0u) {  // as in sweep_kernel
+    if (lds_origin(smem) != 0u || (a.rng_dev && !rng_header_ok(a.rng_dev, a.R))) {  // as in sweep_kernel
         if (tid == 0) a.mag[r] = __builtin_nanf("");
         return;
     }
@@ -860,12 +883,13 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     lat.dw = nullptr;
     lat.rl = 0;
 
-    const uint64_t a_begin = a.a_begin_dev ? *a.a_begin_dev : a.a_begin;
+    const uint64_t a_begin = a.rng_dev ? a.rng_dev->attempts : a.a_begin;
+    const uint64_t seed = a.rng_dev ? a.rng_dev->seed : a.seed;
     const uint32_t b = (uint32_t)(a_begin & 1ull);
     const uint64_t P0 = a_begin >> 1;
     const uint32_t e = (uint32_t)a.n_attempts + b;
     uint32_t l0 = active ? b : e;
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const Geometry<POW2> geo{a.L, a.N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS) drw = a.draws + (size_t)r * (3 * a.n_attempts);
@@ -1189,8 +1213,10 @@ __global__ void gather_spins8(const void *__restrict__ lat, int method, int stri
 __global__ void zero_ints(int *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
-__global__ void advance_counter(uint64_t *c, uint64_t n) {
-    if (threadIdx.x == 0) *c += n;
+// Advances the attempt counter of a device RNG header by n, if the header
+// is one the sweep kernels accept for R replicas (they did not sweep otherwise).
+__global__ void advance_counter(RngHeader *hdr, int R, uint64_t n) {
+    if (threadIdx.x == 0 && rng_header_ok(hdr, R)) hdr->attempts += n;
 }
 
 // ------------------------------------------------------------ checks

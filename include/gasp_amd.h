@@ -280,6 +280,23 @@ int gasp_engine_set_fates(gasp_engine *e, const int *host_fates);
 int gasp_engine_get_fates(gasp_engine *e, int *host_fates);
 /* Largest 4-connected periodic cluster of `spin` per replica (floats). */
 int gasp_engine_largest_cluster(gasp_engine *e, int spin, float *host_out);
+/* Stream-ordered sweeps of caller-owned device grids, the granularity of the
+ * reference driver's per-interval launches (src/mc_gpu.cu:675-708:
+ * mc_sweep_gpu* then compute_magnetisation_gpu) without an engine object.
+ * d_ising_grids (device int32 +-1, ngrids*L*L, as gpuInitGrid makes them)
+ * advance by nsweeps sweeps of gpu_method's kernel at (beta, h) with the
+ * reference's single-precision table; d_mag (device, ngrids floats, or NULL)
+ * receives the magnetisation (float)sum/(float)(L*L).  d_state must come from
+ * gpuInitRand (Philox) with at least ngrids streams: the kernels read its key
+ * and attempt counter from device memory and a kernel advances the counter, so
+ * successive calls continue each replica's chain and nothing is read back to
+ * the host (a state of the wrong kind leaves the grids alone and writes NaN
+ * magnetisations).  The work is queued on `stream` (a hipStream_t; NULL: the
+ * null stream) and not waited for.  Allocates (stream-ordered) a workspace
+ * of about the method's native lattice for each call. */
+int gasp_sweep_grids(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta, double h,
+                     int *d_ising_grids, int nsweeps, float *d_mag, void *stream);
+
 /* Milliseconds spent in sweep kernels on the engine's stream so far. */
 double gasp_engine_kernel_ms(gasp_engine *e);
 /* The engine's HIP stream (hipStream_t). */
