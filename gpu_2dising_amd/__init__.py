@@ -36,6 +36,7 @@ EXPORTS = (
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
     "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream", "gasp_sweep_grids",
+    "gasp_sweep_grids_noalloc",
     "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
     "gasp_bootstrap_errbar_gpu", "gasp_bootstrap_errbar_libc",
     "gasp_cnn_num_params", "gasp_cnn_create", "gasp_cnn_destroy", "gasp_cnn_set_params",
@@ -158,6 +159,8 @@ def lib() -> C.CDLL:
     L.gasp_engine_stream.restype = vp
     L.gasp_sweep_grids.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
     L.gasp_sweep_grids.restype = C.c_int
+    L.gasp_sweep_grids_noalloc.argtypes = L.gasp_sweep_grids.argtypes
+    L.gasp_sweep_grids_noalloc.restype = C.c_int
     dp = C.POINTER(C.c_double)
     L.gasp_graph_create.argtypes = [C.POINTER(vp), C.c_int, ip, ip, ip, dp, dp, C.POINTER(vp)]
     L.gasp_graph_create.restype = C.c_int

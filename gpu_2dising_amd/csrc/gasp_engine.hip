@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -1173,6 +1174,55 @@ extern "C" int gasp_sweep_grids(int L, int ngrids, int gpu_method, gasp_rng_stat
     if (!rc && he != hipSuccess)
         rc = set_error(GASP_ERR_HIP, "GPUassert: %s %s %d", hipGetErrorString(he), __FILE__, __LINE__);
     return rc;
+}
+
+// The same sweeps with the workspace in a file-scope device array instead of
+// a per-call allocation, for callers that cannot allocate.  Each device holds
+// its own copy of the array for the life of the process; its device address
+// is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that address is handed to kernels.  Calls on one device
+// share the array: calls that overlap in time on one device race.  Nothing in
+// the workspace needs to start at zero except the counters, which
+// set_thresholds clears; the lattice is written whole by the pack kernels.
+constexpr size_t kSweepScratchBytes = (size_t)16 << 20;
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kSweepScratchBytes];
+
+static int scratch_for_current_device(unsigned char **out) {
+    static std::mutex mu;
+    static unsigned char *addr[kScratchMaxDevices] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices)
+        return set_error(GASP_ERR_ARG, "gasp_sweep_grids_noalloc: device %d beyond the %d supported", dev,
+                         kScratchMaxDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = (unsigned char *)p;
+    }
+    *out = addr[dev];
+    return GASP_OK;
+}
+
+extern "C" int gasp_sweep_grids_noalloc(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta,
+                                        double h, int *d_ising_grids, int nsweeps, float *d_mag, void *stream) {
+    clear_error();
+    int rc = grids_check_args(L, ngrids, gpu_method, d_state, d_ising_grids, nsweeps);
+    if (rc) return rc;
+    LaunchShape sh;
+    if ((rc = launch_shape(gpu_method, L, ngrids, &sh))) return rc;
+    const size_t bytes = grids_ws_lattice_offset(ngrids, !d_mag) + sh.lat_bytes;
+    if (bytes > kSweepScratchBytes)
+        return set_error(GASP_ERR_ARG,
+                         "gasp_sweep_grids_noalloc: gpu_method %d with L=%d, ngrids=%d needs %zu bytes of "
+                         "workspace, more than the %zu this entry point holds; use gasp_sweep_grids",
+                         gpu_method, L, ngrids, bytes, kSweepScratchBytes);
+    unsigned char *ws = nullptr;
+    if ((rc = scratch_for_current_device(&ws))) return rc;
+    return sweep_grids_run(L, ngrids, gpu_method, d_state, beta, h, d_ising_grids, nsweeps, d_mag,
+                           (hipStream_t)stream, sh, ws);
 }
 
 // ============================================================ GPU bootstrap

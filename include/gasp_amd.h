@@ -296,6 +296,20 @@ int gasp_engine_largest_cluster(gasp_engine *e, int spin, float *host_out);
  * of about the method's native lattice for each call. */
 int gasp_sweep_grids(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta, double h,
                      int *d_ising_grids, int nsweeps, float *d_mag, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  gasp_sweep_grids with the same arguments and results,
+ * but its workspace is a 16 MiB device array inside the library (one per
+ * device, held until the process ends) instead of a per-call allocation; it
+ * calls no hipMalloc/hipFree.  Sizes whose workspace exceeds 16 MiB
+ * (thresholds and counters, the magnetisation when d_mag is NULL, and the
+ * method's native lattice: e.g. method 2 at L = 64 up to about 32000
+ * replicas, method 0 at L = 64 up to 4095) are refused with GASP_ERR_ARG.  All
+ * calls on one device share that array: calls that overlap in time on one
+ * device (other streams or threads) race and must be serialised by the
+ * caller. */
+int gasp_sweep_grids_noalloc(int L, int ngrids, int gpu_method, gasp_rng_state *d_state, double beta, double h,
+                             int *d_ising_grids, int nsweeps, float *d_mag, void *stream);
 
 /* Milliseconds spent in sweep kernels on the engine's stream so far. */
 double gasp_engine_kernel_ms(gasp_engine *e);
