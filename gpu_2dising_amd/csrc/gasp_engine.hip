@@ -18,6 +18,7 @@
 
 #include "gasp_internal.h"
 #include "gasp_kernels.h"
+#include "gasp_sliced.h"
 
 using namespace gasp;
 
@@ -221,6 +222,9 @@ static int pow2_ceil(long long x) {
     return p;
 }
 
+// The native layout of a method: method 3 (bit-sliced) keeps method 2's.
+static int layout_of(int method) { return method == 3 ? 2 : method; }
+
 // Native layout strides.
 static int layout_stride(int method, int N) {
     if (method == 2) return (N + 3) & ~3;         // dwords per plane, two planes per 64-replica group
@@ -263,6 +267,9 @@ static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
 // Chooses slots per replica S and waves per workgroup.  Aim: about four
 // resident waves per SIMD (16 per CU) over the whole device, within the LDS
 // budget; explicit settings from gasp_set_tuning win when they are valid.
+// Method 3 (bit-sliced): LDS of a workgroup (16 bytes per site and a header).
+static size_t sliced_lds_bytes(int N) { return (size_t)kSlLat + (size_t)kSlEntry * (size_t)N; }
+
 // Shapes picked by gasp_autotune, keyed by (method, L, ngrids); used when no
 // explicit gasp_set_tuning override is active.
 static std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;
@@ -346,9 +353,41 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     *W_out = W;
 }
 
+// Method 3: waves per workgroup (*S_out: 1, 2, 4, 8) and word split
+// (*W_out: 1 = 64 replicas per word, 2 = 32).  Default: whole words, and
+// waves per workgroup until there are about four waves per CU.
+static void choose_sliced(int L, int R, int *S_out, int *W_out) {
+    if (g_tune_slots <= 0 && g_tune_waves <= 0) {
+        auto it = g_tuned.find(std::make_tuple(3, L, R));
+        if (it != g_tuned.end()) {
+            *S_out = it->second.first;
+            *W_out = it->second.second;
+            return;
+        }
+    }
+    DeviceInfo d = device_info();
+    const int ncu = d.ok ? d.ncu : 256;
+    const int split = (g_tune_waves == 2) ? 2 : 1;
+    const long long groups = (R + 64 / split - 1) / (64 / split);
+    int S = 1;
+    if (g_tune_slots > 0) {
+        S = std::min(pow2_ceil(g_tune_slots), 8);
+    } else {
+        while (S < 8 && groups * S < 4LL * ncu) S *= 2;
+    }
+    *S_out = S;
+    *W_out = split;
+}
+
 extern "C" void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
                                 int *waves_per_group) {
     int S = 1, W = 1;
+    if (method == 3) {
+        choose_sliced(L, ngrids, &S, &W);
+        if (slots_per_replica) *slots_per_replica = S;
+        if (waves_per_group) *waves_per_group = W;
+        return;
+    }
     choose_tuning(method, L, ngrids, &S, &W);
     if (slots_per_replica) *slots_per_replica = S;
     if (waves_per_group) *waves_per_group = W;
@@ -390,7 +429,25 @@ static SweepFn pick_sweep2(int S, bool pow2) {
     return pow2 ? pick_sweep<LAT, RNG, true>(S) : pick_sweep<LAT, RNG, false>(S);
 }
 
+template <int WB, bool P2>
+static SweepFn pick_sliced(int waves) {
+    switch (waves) {
+        case 1: return sweep_sliced<WB, 1, P2>;
+        case 2: return sweep_sliced<WB, 2, P2>;
+        case 4: return sweep_sliced<WB, 4, P2>;
+        case 8: return sweep_sliced<WB, 8, P2>;
+        default: return nullptr;
+    }
+}
+
+// Method 3: `waves` per workgroup, `split` 1 (64 replicas per word) or 2 (32).
+static SweepFn sliced_fn(int waves, int split, bool pow2) {
+    if (split == 2) return pow2 ? pick_sliced<32, true>(waves) : pick_sliced<32, false>(waves);
+    return pow2 ? pick_sliced<64, true>(waves) : pick_sliced<64, false>(waves);
+}
+
 static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2) {
+    if (method == 3) return rng == 0 ? sliced_fn(S, W, pow2) : nullptr;
     if (method == 2 && W == 2)
         return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2);
     if (method == 2 && W == 4)
@@ -420,10 +477,30 @@ struct LaunchShape {
     int tbl = 1;  // sweep_hbm_waves: conflict-table entries
     int ncu = 256;
     size_t lds = 0, lat_bytes = 0;
+    bool sliced = false;  // method 3: sweep_sliced, S = waves per workgroup, W = word split (1: 64, 2: 32)
 };
 
 static int launch_shape(int method, int L, int R, LaunchShape *sh) {
     const int N = L * L;
+    if (method == 3) {
+        // bit-sliced multispin: the method-2 layout in HBM; one workgroup of
+        // S waves per word group of 64 (W = 1) or 32 (W = 2) replicas
+        choose_sliced(L, R, &sh->S, &sh->W);
+        sh->sliced = true;
+        sh->stride = layout_stride(2, N);
+        DeviceInfo di = device_info();
+        if (di.ok) sh->ncu = di.ncu;
+        sh->lds = sliced_lds_bytes(N);
+        sh->rpw = 64 / sh->W;
+        sh->block = 64 * sh->S;
+        sh->nblocks = (R + sh->rpw - 1) / sh->rpw;
+        sh->lat_bytes = (size_t)((R + 63) / 64) * sh->stride * 8;
+        if (di.ok && sh->lds > di.lds_block)
+            return set_error(GASP_ERR_ARG,
+                             "Error: gpu_method 3 needs %zu bytes of LDS per workgroup at L=%d (device limit %zu)",
+                             sh->lds, L, (size_t)di.lds_block);
+        return GASP_OK;
+    }
     choose_tuning(method, L, R, &sh->S, &sh->W);
     sh->stride = layout_stride(method, N);
     DeviceInfo di = device_info();
@@ -508,7 +585,7 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     *out = nullptr;
     if (L < 2 || L > 8192) return set_error(GASP_ERR_ARG, "Error: L must lie in [2, 8192]");
     if (R < 1) return set_error(GASP_ERR_ARG, "Error: ngrids must be positive");
-    if (method < 0 || method > 2) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
+    if (method < 0 || method > 3) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
     if (!d_state) return set_error(GASP_ERR_ARG, "Error: RNG state is NULL (call gpuInitRand)");
     gasp_engine *e = new gasp_engine();
     e->L = L;
@@ -528,6 +605,10 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     if (e->rng.ngrids < R) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: RNG state holds %d streams, need %d", e->rng.ngrids, R);
+    }
+    if (e->rng.kind == KIND_MT && method == 3) {
+        engine_free(e);
+        return set_error(GASP_ERR_ARG, "Error: gpu_method 3 (bit-sliced) draws from Philox streams (gpuInitRand)");
     }
     if (e->rng.kind == KIND_MT && e->rng.ngrids != R) {
         engine_free(e);
@@ -742,7 +823,7 @@ static int engine_set_grids_impl(gasp_engine *e, const int *grids, int on_device
     }
     const int N = e->N, R = e->R, per = R / ninputs;
     const dim3 grid(1024), blk(256);
-    if (e->method == 2) {
+    if (layout_of(e->method) == 2) {
         hipLaunchKernelGGL(pack_multispin, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride, per);
     } else if (e->method == 1) {
         hipLaunchKernelGGL(pack_bits, grid, blk, 0, e->stream, src, (unsigned *)e->d_lat, R, N, e->stride, per);
@@ -757,7 +838,7 @@ template <typename T>
 static int engine_unpack(gasp_engine *e, T *dst) {
     const int N = e->N, R = e->R;
     const dim3 grid(1024), blk(256);
-    if (e->method == 2) {
+    if (layout_of(e->method) == 2) {
         hipLaunchKernelGGL(unpack_multispin<T>, grid, blk, 0, e->stream, (const unsigned *)e->d_lat, dst, R, N,
                            e->stride);
     } else if (e->method == 1) {
@@ -952,7 +1033,7 @@ extern "C" int gasp_engine_get_gridstates(gasp_engine *e, uint8_t *host_out) {
     uint8_t *d = nullptr;
     hipError_t he = hipMalloc(&d, nbytes);
     if (he == hipSuccess) {
-        hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+        hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, layout_of(e->method),
                            e->stride, e->R, e->N, d, nbytes);
         he = hipGetLastError();
     }
@@ -981,7 +1062,7 @@ extern "C" int gasp_engine_gather_grids(gasp_engine *e, const int *replicas, int
     if (he == hipSuccess) he = hipMalloc(&d_out, bytes);
     if (he == hipSuccess) he = hipMemcpyAsync(d_idx, replicas, (size_t)n * sizeof(int), hipMemcpyHostToDevice, e->stream);
     if (he == hipSuccess) {
-        hipLaunchKernelGGL(gather_spins8, dim3(1024), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+        hipLaunchKernelGGL(gather_spins8, dim3(1024), dim3(256), 0, e->stream, (const void *)e->d_lat, layout_of(e->method),
                            e->stride, d_idx, n, e->N, d_out);
         he = hipGetLastError();
     }
@@ -1084,7 +1165,7 @@ static size_t grids_ws_lattice_offset(int R, bool own_mag) {
 static int grids_check_args(int L, int R, int method, gasp_rng_state *d_state, const int *d_grids, int nsweeps) {
     if (L < 2 || L > 8192) return set_error(GASP_ERR_ARG, "Error: L must lie in [2, 8192]");
     if (R < 1) return set_error(GASP_ERR_ARG, "Error: ngrids must be positive");
-    if (method < 0 || method > 2) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
+    if (method < 0 || method > 3) return set_error(GASP_ERR_ARG, "Unknown gpu_method in ising.cu");
     if (!d_state) return set_error(GASP_ERR_ARG, "Error: RNG state is NULL (call gpuInitRand)");
     if (!d_grids) return set_error(GASP_ERR_ARG, "Error: device grids are NULL");
     if (nsweeps < 0) return set_error(GASP_ERR_ARG, "Error: nsweeps must not be negative");
@@ -1106,9 +1187,10 @@ static int sweep_grids_run(int L, int R, int method, gasp_rng_state *d_state, do
     hipLaunchKernelGGL(set_thresholds, dim3(1), dim3(64), 0, st, t, d_thr, d_counts);
     HIP_TRY(hipGetLastError());
     const dim3 grid(1024), blk(256);
-    if (method == 2)
+    const int lay = layout_of(method);
+    if (lay == 2)
         hipLaunchKernelGGL(pack_multispin, grid, blk, 0, st, d_grids, (unsigned *)lat, R, N, sh.stride, 1);
-    else if (method == 1)
+    else if (lay == 1)
         hipLaunchKernelGGL(pack_bits, grid, blk, 0, st, d_grids, (unsigned *)lat, R, N, sh.stride, 1);
     else
         hipLaunchKernelGGL(pack_hbm, grid, blk, 0, st, d_grids, (unsigned char *)lat, R, N, sh.stride, 1);
@@ -1145,9 +1227,9 @@ static int sweep_grids_run(int L, int R, int method, gasp_rng_state *d_state, do
         }
         left -= nb;
     } while (left > 0);
-    if (method == 2)
+    if (lay == 2)
         hipLaunchKernelGGL(unpack_multispin<int>, grid, blk, 0, st, (const unsigned *)lat, d_grids, R, N, sh.stride);
-    else if (method == 1)
+    else if (lay == 1)
         hipLaunchKernelGGL(unpack_bits<int>, grid, blk, 0, st, (const unsigned *)lat, d_grids, R, N, sh.stride);
     else
         hipLaunchKernelGGL(unpack_hbm<int>, grid, blk, 0, st, (const unsigned char *)lat, d_grids, R, N, sh.stride);
@@ -1485,7 +1567,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
                              int check_sweeps, int *best_slots, int *best_waves, int *best_launch_sweeps,
                              double *best_rate) {
     clear_error();
-    if (method < 0 || method > 2 || L < 2 || ngrids < 1 || probe_sweeps < 1 || check_sweeps < 0)
+    if (method < 0 || method > 3 || L < 2 || ngrids < 1 || probe_sweeps < 1 || check_sweeps < 0)
         return set_error(GASP_ERR_ARG, "gasp_autotune: bad argument");
     const int save_s = g_tune_slots, save_w = g_tune_waves;
     std::vector<std::pair<int, int>> cand;
@@ -1494,7 +1576,13 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
     // half-word workgroups per CU)
     const bool few = (long long)((ngrids + 63) / 64) * 2 < 2LL * ncu_t;
     for (int S = 1; S <= 64; S <<= 1) {
-        if (method == 2) {
+        if (method == 3) {
+            // waves per workgroup x word split
+            if (S <= 8) {
+                cand.push_back({S, 1});
+                cand.push_back({S, 2});
+            }
+        } else if (method == 2) {
             if (S <= 16) cand.push_back({S, 1});
             if (S >= 2 && S <= 32) cand.push_back({S, 2});
             if (few && S >= 8 && S <= 32) {
@@ -1521,7 +1609,8 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
     int bs = 1, bw = 1;
     DeviceInfo di = device_info();
     for (auto [S, W] : cand) {
-        if (di.ok && lds_bytes_for(method, L * L, S, W, ngrids) > di.lds_block) continue;
+        const size_t lds = method == 3 ? sliced_lds_bytes(L * L) : lds_bytes_for(method, L * L, S, W, ngrids);
+        if (di.ok && lds > di.lds_block) continue;
         g_tune_slots = S;
         g_tune_waves = W;
         gasp_engine *e = nullptr;
@@ -1755,7 +1844,7 @@ extern "C" void mc_driver_gpu(mc_gpu_grids_t grids, double beta, double h, int *
             if ((rc = engine_check_cv(e, e->d_lclus, check_mode, dn, up))) break;
         }
         if (do_grid && packed_out) {
-            hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, e->method,
+            hipLaunchKernelGGL(pack_gridstates, dim3(2048), dim3(256), 0, e->stream, (const void *)e->d_lat, layout_of(e->method),
                                e->stride, R, e->N, d_bits, pbytes);
             if (hipGetLastError() != hipSuccess ||
                 hipMemcpyAsync(h_bits, d_bits, pbytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess) {

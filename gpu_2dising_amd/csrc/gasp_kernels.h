@@ -141,7 +141,8 @@ __device__ __forceinline__ uint64_t wave_mask(bool p) { return __builtin_amdgcn_
 //  which for L = 2^lg is u >> (32 - lg).
 // Stencil positions are site indices times the geometry's unit U: U = 1 for
 // site indices, U = 4 for byte offsets of a dword-per-site LDS lattice (the
-// split multispin layout), which then serve as LDS addresses directly.
+// split multispin layout), U = 16 for the 16-byte site entries of the
+// bit-sliced kernel; the offsets then serve as LDS addresses directly.
 // row and col are the site's row and column mod 64, all the conflict filter
 // uses.
 struct Stencil {
@@ -160,8 +161,9 @@ __device__ __forceinline__ uint64_t ballot_in_stencil(int x, const Stencil &s) {
 
 template <bool POW2, int U = 1>
 struct Geometry {
-    static_assert(U == 1 || U == 2 || U == 4, "positions in sites or in byte offsets of 2- or 4-byte entries");
-    static constexpr int kLgU = U == 4 ? 2 : (U == 2 ? 1 : 0);
+    static_assert(U == 1 || U == 2 || U == 4 || U == 16,
+                  "positions in sites or in byte offsets of 2-, 4- or 16-byte entries");
+    static constexpr int kLgU = U == 16 ? 4 : (U == 4 ? 2 : (U == 2 ? 1 : 0));
     int L, N, lg;  // lg = log2(L) when POW2
 
     __device__ __forceinline__ Stencil from_rowcol(int row, int col) const {

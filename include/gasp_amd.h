@@ -217,8 +217,10 @@ void gasp_free(void *d_ptr);
  * halves, or smaller parts when that leaves fewer workgroups than CUs or
  * when the lattice would not fit the LDS otherwise.
  * Method 0 also takes 128, 256 or 512 slots: one replica per workgroup, spread
- * over 2, 4 or 8 waves (the waves value is then implied).  Other methods: up
- * to 64.
+ * over 2, 4 or 8 waves (the waves value is then implied).  Methods 1 and 2: up
+ * to 64.  Method 3 (bit-sliced, below): the first value is the number of
+ * waves per workgroup (1, 2, 4 or 8) that evaluate one word group's attempts
+ * together, the second the word split (1: 64 replicas per word, 2: 32).
  * Applies to subsequent runs. */
 void gasp_set_tuning(int slots_per_replica, int waves_per_group);
 void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
@@ -239,7 +241,19 @@ void gasp_get_run_stats(gasp_run_stats_t *out);
 
 /* Stream-level engine (used by benchmarks and by callers that drive the
  * schedule themselves).  The engine owns the replicas' lattice in the
- * method's native layout.  All calls are ordered on the engine's stream. */
+ * method's native layout.  All calls are ordered on the engine's stream.
+ *
+ * gpu_method 3 (engine-level only; mc_driver_gpu, the CLI and the Python
+ * module keep the reference's 0/1/2): bit-sliced multispin.  The replicas of
+ * a 64-bit word (or of a 32-bit half, see gasp_set_tuning) share one random
+ * site sequence; every attempt updates all of them at once with bitwise
+ * adders for the neighbour sums and a bit-sliced comparison of each
+ * replica's own 32-bit acceptance draw with its class threshold (the
+ * single-precision table).  Each replica follows random-site Metropolis;
+ * the replicas of a word are correlated through their common sites.
+ * Philox states only (gpuInitRand).  Layout, grids, dumps and checks as
+ * method 2.  With freezing, a replica with a decided fate keeps its spins
+ * while the rest of its word continues. */
 typedef struct gasp_engine gasp_engine;
 int gasp_engine_create(int L, int ngrids, int gpu_method, gasp_rng_state *d_state,
                        double beta, double h, gasp_engine **out);

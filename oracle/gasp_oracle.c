@@ -172,6 +172,124 @@ void orc_sweep_philox(int L, int32_t *g, uint64_t seed, uint32_t replica,
     }
 }
 
+/* ====================================================== bit-sliced stream */
+/* The engine's bit-sliced multispin mode (gpu_method 3, gasp_sliced.h),
+ * restated per replica.  Group `group` holds replicas wb*group + b,
+ * b = 0 .. nrep-1 (wb = 64 or 32), grids g0 + b*L*L.  Attempt a of the group:
+ * Philox4x32-10 block k = philox(ctr = (a_lo, a_hi, group, ORC_SLICED_DOMAIN
+ * + k), key = seed) holds words 4k .. 4k+3 of the attempt's word stream; the
+ * site of every replica is floor(w_0 N / 2^32) (the production stream's site
+ * map); bit 31 - p of replica b's 32-bit acceptance draw u is bit (b & 31)
+ * of word (wb/32)(p + 1) + b/32 (plane p); replica b flips iff u < T of its
+ * class, the single-precision table's threshold at the reference index
+ * 5(s+1)+nsum+4 (src/mc_cpu.c:55-84 arithmetic; T >= 2^32 always flips).
+ * Replicas with frozen[b] set do not change.  The whole draw is assembled for
+ * every replica; the GPU stops at the first differing bit, which decides the
+ * same comparison. */
+void orc_sweep_sliced(int L, int32_t *g0, int nrep, int wb, uint64_t seed, uint32_t group,
+                      uint64_t attempt0, long long nattempts, const uint64_t thr[20],
+                      const uint8_t *frozen)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint64_t N = (uint64_t)L * (uint64_t)L;
+    const int q = wb / 32;
+    uint32_t blk[17][4];
+    for (long long i = 0; i < nattempts; i++) {
+        uint64_t a = attempt0 + (uint64_t)i;
+        int nblk = 0;
+        for (; nblk < 17; nblk++) {
+            uint32_t ctr[4] = {(uint32_t)a, (uint32_t)(a >> 32), group, ORC_SLICED_DOMAIN + (uint32_t)nblk};
+            orc_philox4x32_10(ctr, key, blk[nblk]);
+        }
+        int site = (int)(((uint64_t)blk[0][0] * N) >> 32);
+        int row = site / L, col = site % L;
+        int up = L * ((row + L - 1) % L) + col;
+        int dn = L * ((row + 1) % L) + col;
+        int rt = L * row + (col + 1) % L;
+        int lt = L * row + (col + L - 1) % L;
+        for (int b = 0; b < nrep; b++) {
+            if (frozen && frozen[b]) continue;
+            int32_t *g = g0 + (long long)b * (long long)N;
+            int nsum = g[up] + g[dn] + g[lt] + g[rt];
+            int idx = 5 * (g[site] + 1) + nsum + 4;
+            int flip;
+            if (thr[idx] >= ((uint64_t)1 << 32)) {
+                flip = 1;
+            } else {
+                uint32_t u = 0;
+                for (int p = 0; p < 32; p++) {
+                    int word = q * (p + 1) + b / 32;
+                    uint32_t bit = (blk[word >> 2][word & 3] >> (b & 31)) & 1u;
+                    u |= bit << (31 - p);
+                }
+                flip = (uint64_t)u < thr[idx];
+            }
+            if (flip) g[site] = -g[site];
+        }
+    }
+}
+
+typedef struct {
+    int L, R, wb, g0, g1;
+    int32_t *grids;
+    uint64_t seed, attempt0;
+    long long nattempts;
+    const uint64_t *thr;
+    const uint8_t *frozen;
+} sliced_job_t;
+
+static void *sliced_worker(void *arg)
+{
+    sliced_job_t *j = (sliced_job_t *)arg;
+    long long N = (long long)j->L * j->L;
+    for (int g = j->g0; g < j->g1; g++) {
+        int r0 = g * j->wb;
+        int nrep = j->R - r0 < j->wb ? j->R - r0 : j->wb;
+        orc_sweep_sliced(j->L, j->grids + (long long)r0 * N, nrep, j->wb, j->seed, (uint32_t)g, j->attempt0,
+                         j->nattempts, j->thr, j->frozen ? j->frozen + r0 : NULL);
+    }
+    return NULL;
+}
+
+double orc_sweep_sliced_batch(int L, int R, int32_t *grids, int wb, uint64_t seed, uint64_t attempt0,
+                              long long nattempts, double beta, double h, const uint8_t *frozen, int nthreads)
+{
+    float paccf[20];
+    uint64_t thr[20];
+    orc_pacc_gpu(beta, h, paccf);
+    for (int i = 0; i < 20; i++) thr[i] = orc_threshold((double)paccf[i]);
+    int ngroups = (R + wb - 1) / wb;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ngroups) nthreads = ngroups;
+    pthread_t tid[256];
+    sliced_job_t jobs[256];
+    if (nthreads > 256) nthreads = 256;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int t = 0; t < nthreads; t++) {
+        sliced_job_t *j = &jobs[t];
+        j->L = L;
+        j->R = R;
+        j->wb = wb;
+        j->g0 = (int)((long long)ngroups * t / nthreads);
+        j->g1 = (int)((long long)ngroups * (t + 1) / nthreads);
+        j->grids = grids;
+        j->seed = seed;
+        j->attempt0 = attempt0;
+        j->nattempts = nattempts;
+        j->thr = thr;
+        j->frozen = frozen;
+    }
+    if (nthreads == 1) {
+        sliced_worker(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; t++) pthread_create(&tid[t], NULL, sliced_worker, &jobs[t]);
+        for (int t = 0; t < nthreads; t++) pthread_join(tid[t], NULL);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
 /* ========================================================= collective vars */
 float orc_magnetisation_cpu(int L, const int32_t *g)
 {

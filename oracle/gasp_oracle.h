@@ -122,6 +122,19 @@ double orc_sweep_batch(int L, int R, int32_t *grids, int rng_kind, orc_mt_t *mt,
                        uint64_t seed, uint64_t attempt0, long long nattempts,
                        double beta, double h, int nthreads);
 
+/* The engine's bit-sliced multispin stream (gpu_method 3): one group of nrep
+ * replicas (wb = 64 or 32 per group, group index `group`) sharing a site
+ * sequence, each replica's acceptance draw assembled from random bit planes;
+ * see gasp_oracle.c.  frozen: nrep flags (or NULL), frozen replicas do not
+ * change.  The batch form runs every group of R replicas on threads with the
+ * single-precision table at (beta, h); frozen has R flags or is NULL. */
+#define ORC_SLICED_DOMAIN 0x534C4300u
+void orc_sweep_sliced(int L, int32_t *g0, int nrep, int wb, uint64_t seed, uint32_t group,
+                      uint64_t attempt0, long long nattempts, const uint64_t thr[20],
+                      const uint8_t *frozen);
+double orc_sweep_sliced_batch(int L, int R, int32_t *grids, int wb, uint64_t seed, uint64_t attempt0,
+                              long long nattempts, double beta, double h, const uint8_t *frozen, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,11 @@ def lib():
     L.orc_sweep_batch.argtypes = [C.c_int, C.c_int, _i32p, C.c_int, C.c_void_p, C.c_uint64,
                                   C.c_uint64, C.c_longlong, C.c_double, C.c_double, C.c_int]
     L.orc_sweep_batch.restype = C.c_double
+    L.orc_sweep_sliced.argtypes = [C.c_int, _i32p, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64,
+                                   C.c_longlong, _u64p, C.c_void_p]
+    L.orc_sweep_sliced_batch.argtypes = [C.c_int, C.c_int, _i32p, C.c_int, C.c_uint64, C.c_uint64,
+                                         C.c_longlong, C.c_double, C.c_double, C.c_void_p, C.c_int]
+    L.orc_sweep_sliced_batch.restype = C.c_double
     L.orc_driver.argtypes = [C.c_int, C.c_int, _i32p, C.c_int, C.c_uint64, C.c_double, C.c_double,
                              C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                              C.c_double, C.c_double, C.c_int, C.c_void_p, _f32p,
@@ -150,6 +155,23 @@ def sweep_mt_per_replica(L: int, grids: np.ndarray, seed: int, nsweeps: int, bet
     flat = np.ascontiguousarray(grids.reshape(R, -1), dtype=np.int32)
     lib().orc_sweep_batch(L, R, flat, 0, C.cast(states, C.c_void_p), 0, 0,
                           nsweeps * L * L, beta, h, nthreads)
+    grids[...] = flat.reshape(grids.shape)
+    return grids
+
+
+def sweep_sliced(L: int, grids: np.ndarray, wb: int, seed: int, attempt0: int, nattempts: int,
+                 beta: float, h: float, frozen=None, nthreads: int = 8) -> np.ndarray:
+    """The bit-sliced multispin stream (engine gpu_method 3, wb = 64 or 32
+    replicas per word group): every group advanced by nattempts attempts from
+    attempt index attempt0, in place; frozen (R booleans or None) replicas do
+    not change."""
+    R = grids.shape[0]
+    flat = np.ascontiguousarray(grids.reshape(R, -1), dtype=np.int32)
+    fz = None
+    if frozen is not None:
+        fz = np.ascontiguousarray(np.asarray(frozen, dtype=np.uint8).reshape(R))
+    lib().orc_sweep_sliced_batch(L, R, flat, wb, seed, attempt0, nattempts, beta, h,
+                                 None if fz is None else fz.ctypes.data_as(C.c_void_p), nthreads)
     grids[...] = flat.reshape(grids.shape)
     return grids
 

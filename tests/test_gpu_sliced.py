@@ -1,0 +1,172 @@
+"""The bit-sliced multispin mode (engine gpu_method 3) against the oracle's
+restatement of its random stream (oracle.sweep_sliced: per replica,
+random-site Metropolis with the site shared by the replicas of a word and the
+acceptance draw assembled from bit planes; src/mc_cpu.c:41-89 arithmetic with
+the single-precision table of src/mc_gpu.cu:21-37), spin for spin:
+
+  - 64- and 32-replica words, 1, 2, 4 and 8 waves per workgroup, lattices from
+    L = 2 (neighbours coincide) to 64, partial last words, random half-accepting
+    starts (many dependent attempts per window) and the metastable start;
+  - launches that continue one chain, zero-sweep launches (magnetisation only);
+  - frozen replicas with a decided fate keep their spins, the rest of the word
+    continues; fates and counts of the threshold check;
+  - a captured graph and the caller-grid entry point with method 3.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BETA, H = 0.54, 0.07
+
+
+def _start(kind, R, L, seed):
+    if kind == "down":
+        return -np.ones((R, L, L), dtype=np.int32)
+    rs = np.random.default_rng(seed)
+    return np.where(rs.random((R, L, L)) < 0.5, 1, -1).astype(np.int32)
+
+
+def _run(G, L, R, waves, split, g0, seed, beta, h, chunks):
+    G.set_tuning(waves, split)
+    try:
+        rng = G.RngState(R, seed)
+        with G.Engine(L, R, 3, rng, beta, h) as e:
+            assert G.get_tuning(3, L, R) == (waves, split)
+            e.set_grids(g0)
+            for n in chunks:
+                e.sweep(n)
+            out = e.get_grids()
+            mag = e.magnetisation()
+        rng.free()
+    finally:
+        G.set_tuning(0, 0)
+    return out, mag
+
+
+CASES = [(w, s) for s in (1, 2) for w in (1, 2, 4, 8)]
+
+
+@pytest.mark.parametrize("waves,split", CASES)
+@pytest.mark.parametrize("L,R", [(16, 100), (17, 64), (8, 40)])
+def test_sliced_random_start_bit_exact(gpu, engine_lib, oracle, waves, split, L, R):
+    G, O = engine_lib, oracle
+    seed, beta, wb = 8675309 + L, 0.3, 64 // split
+    print(f"[sliced] random start waves={waves} wb={wb} L={L} R={R}", flush=True)
+    g0 = _start("random", R, L, seed)
+    got, mag = _run(G, L, R, waves, split, g0, seed, beta, H, [3, 0, 2])
+    ref = O.sweep_sliced(L, g0.copy(), wb, seed, 0, 5 * L * L, beta, H)
+    assert not np.array_equal(ref, g0)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(mag, O.magnetisation_gpu(ref))
+
+
+@pytest.mark.parametrize("waves,split", [(1, 1), (4, 1), (2, 2), (8, 2)])
+@pytest.mark.parametrize("L", [2, 3, 5, 32, 64])
+def test_sliced_metastable_bit_exact(gpu, engine_lib, oracle, waves, split, L):
+    G, O = engine_lib, oracle
+    R, seed, wb = 130, 4242 + L, 64 // split
+    print(f"[sliced] metastable waves={waves} wb={wb} L={L} R={R}", flush=True)
+    g0 = _start("down", R, L, seed)
+    nsw = 40 if L <= 5 else 6
+    got, mag = _run(G, L, R, waves, split, g0, seed, BETA, H, [nsw // 2, nsw - nsw // 2])
+    ref = O.sweep_sliced(L, g0.copy(), wb, seed, 0, nsw * L * L, BETA, H)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(mag, O.magnetisation_gpu(ref))
+
+
+def test_sliced_negative_field_and_zero_field(gpu, engine_lib, oracle):
+    """h < 0 and h = 0 change which classes have thresholds below 2^32 (the
+    down spins with two up neighbours, or no class with nup = 2)."""
+    G, O = engine_lib, oracle
+    L, R = 16, 128
+    for h in (-0.3, 0.0, 2.5):
+        print(f"[sliced] h={h}", flush=True)
+        g0 = _start("random", R, L, 77)
+        got, _ = _run(G, L, R, 2, 1, g0, 77, 0.45, h, [4])
+        ref = O.sweep_sliced(L, g0.copy(), 64, 77, 0, 4 * L * L, 0.45, h)
+        assert np.array_equal(got, ref), h
+
+
+def test_sliced_freeze_and_checks(gpu, engine_lib, oracle):
+    G, O = engine_lib, oracle
+    L, R, seed, wb = 16, 192, 99, 64
+    dn, up = -0.2, 0.2
+    print("[sliced] freeze + fates", flush=True)
+    g0 = _start("random", R, L, seed)
+    rs = np.random.default_rng(5)
+    fates = np.where(rs.random(R) < 0.3, rs.integers(0, 2, R), -1).astype(np.int32)
+    rng = G.RngState(R, seed)
+    ref = g0.copy()
+    ofates = fates.copy()
+    with G.Engine(L, R, 3, rng, 0.35, H) as e:
+        e.set_grids(g0)
+        e.set_fates(fates)
+        e.set_freeze(True)
+        a = 0
+        for n in (2, 3, 1):
+            e.sweep(n, G.CHECK_FATES, dn, up)
+            O.sweep_sliced(L, ref, wb, seed, a, n * L * L, 0.35, H, frozen=ofates != -1)
+            a += n * L * L
+            m = O.magnetisation_gpu(ref).astype(np.float64)
+            und = ofates == -1
+            ofates[und & (m > up)] = 1
+            ofates[und & (m < dn) & ~(m > up)] = 0
+            assert np.array_equal(e.get_grids(), ref)
+            assert np.array_equal(e.fates(), ofates)
+            c = e.counts()
+            assert c[1] == int((ofates == 0).sum()) and c[2] == int((ofates == 1).sum())
+        # replicas decided at the start never moved
+        assert np.array_equal(e.get_grids()[fates != -1], g0[fates != -1])
+    rng.free()
+
+
+def test_sliced_graph_replays(gpu, engine_lib, oracle):
+    G, O = engine_lib, oracle
+    L, R, seed = 32, 96, 31
+    print("[sliced] graph replays", flush=True)
+    g0 = _start("random", R, L, seed)
+    rng = G.RngState(R, seed)
+    e = G.Engine(L, R, 3, rng, 0.4, H)
+    e.set_grids(g0)
+    graph = G.Graph([e], [3], [1], [-0.9], [0.1], [1])
+    ref = g0.copy()
+    for rep in range(3):
+        graph.launch(1)
+        O.sweep_sliced(L, ref, 64, seed, rep * 3 * L * L, 3 * L * L, 0.4, H)
+        assert np.array_equal(e.get_grids(), ref), rep
+        assert e.counts()[0] == int((O.magnetisation_gpu(ref).astype(np.float64) > 0.1).sum())
+    graph.close()
+    e.close()
+    rng.free()
+
+
+def test_sliced_caller_grids(gpu, engine_lib, oracle):
+    """gasp_sweep_grids with method 3 on int32 device grids, two calls that
+    continue one chain (the counter in the RNG state's device header)."""
+    from test_gpu_grids import Hip
+
+    G, O = engine_lib, oracle
+    lib = G.lib()
+    hip = Hip()
+    L, R, seed = 24, 150, 555
+    print("[sliced] caller grids", flush=True)
+    g0 = _start("random", R, L, seed)
+    d_g, d_m = hip.alloc(g0.nbytes), hip.alloc(R * 4)
+    hip.h2d(d_g, g0)
+    rng = G.RngState(R, seed)
+    ref, a = g0.copy(), 0
+    for nsw in (2, 3):
+        G.check(lib.gasp_sweep_grids(L, R, 3, rng.ptr, 0.4, H, d_g, nsw, d_m, None))
+        hip.sync()
+        O.sweep_sliced(L, ref, 64, seed, a, nsw * L * L, 0.4, H)
+        a += nsw * L * L
+        got = np.empty_like(g0)
+        hip.d2h(got, d_g)
+        mag = np.empty(R, dtype=np.float32)
+        hip.d2h(mag, d_m)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(mag, O.magnetisation_gpu(ref))
+    rng.free()
+    hip.free(d_g)
+    hip.free(d_m)
