@@ -16,9 +16,12 @@ steps (wall clock, HIP-synchronised on both sides).  `roofline` is for the
 sweep kernel, timed with HIP events on the engine's stream: LDS bytes per the
 algorithmic figure, with the VALU issue rate beside it and the limiting one
 named.  `cpu_baseline` times the reference's own CPU sweep (src/mc_cpu.c,
-compiled from its sources into oracle/_ref) on up to 16 host cores over a
-bounded sample, with one core and the oracle's restatement reported beside it.
-The committor, surrogate and batched legs follow the throughput leg.
+compiled from its sources into oracle/_ref) on every host core this process
+may use (the host's nproc, affinity set, cgroup quota and lscpu model are
+recorded) over a bounded sample, with one core and the oracle's restatement
+reported beside it.  The committor, surrogate and batched legs follow the
+throughput leg; the committor leg carries the reference's CPU committor
+measured to completion on one input per host core.
 """
 from __future__ import annotations
 
@@ -128,36 +131,83 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
 
 
 SURROGATE_PARAMS = os.path.join(ROOT, "gpu_2dising_amd", "surrogate_L64.npz")
+# trained on 20000 nucleation-trajectory grids only (the training set as the surrogate workload states it)
+SURROGATE_PARAMS_TRAJ = os.path.join(ROOT, "gpu_2dising_amd", "surrogate_L64_traj.npz")
+
+
+def _training_cost(path: str, blob) -> dict:
+    """GPU seconds of the training data's labels and of the training (stored with the parameters,
+    or in the summary written beside them)."""
+    out = {k: float(blob[k]) for k in ("val_mae", "n_train", "label_seconds", "train_seconds", "n_trajectory",
+                                        "n_discs") if k in blob.files}
+    js = path[:-4] + ".json"
+    if "label_seconds" not in out and os.path.exists(js):
+        with open(js) as f:
+            meta = json.load(f)
+        for k in ("label_seconds", "train_seconds", "n_trajectory", "n_discs"):
+            if k in meta:
+                out[k] = float(meta[k])
+    return out
 
 
 def surrogate_leg(G, shooting: dict, n_inputs: int = 512, fallback_reps: int = 128) -> dict:
-    """Committor estimates for the same 512 inputs as the shooting leg from the
-    trained network, with shooting (fallback_reps replicas) for predictions in
-    [0.2, 0.8].  MAE is measured against the shooting leg's 1024-replica pB."""
+    """Committor estimates for the shooting leg's 512 inputs from the trained network, with shooting
+    (fallback_reps replicas) for predictions in [0.2, 0.8]; beside it plain shooting of every input at
+    16 .. 512 replicas.  Accuracy is measured against the shooting leg's 1024-replica pB (MAE over
+    all inputs and over those with pB in [0.2, 0.8]); the gain is over plain shooting at the smallest
+    replica count that reaches the surrogate path's MAE, and the training cost (labels + training,
+    GPU seconds) is reported per estimate of this run and as the break-even number of estimates."""
     from gpu_2dising_amd import surrogate as SG
 
-    blob = np.load(SURROGATE_PARAMS)
-    params = blob["params"]
     inputs = committor_inputs(n_inputs, L).astype(np.int8)
     ref = shooting["_pB"]
-    t0 = time.perf_counter()
-    est, unsure, tm = SG.estimate(inputs, params, reps=fallback_reps, seed=4040)
-    secs = time.perf_counter() - t0
     ok = ~np.isnan(ref)
-    err = np.abs(est - ref)[ok]
-    return {
-        "metric": "committor pB estimates/sec (surrogate, shooting fallback for pB in [0.2, 0.8])",
-        "value": n_inputs / secs, "unit": "pB estimates/s",
-        "mae_vs_shooting": float(err.mean()),
-        "mae_surrogate_only": float(np.abs(est - ref)[ok & ~unsure].mean()) if (ok & ~unsure).any() else None,
-        "fallback_fraction": float(unsure.mean()), "fallback_replicas": fallback_reps,
-        "seconds": secs, "predict_seconds": tm["predict_seconds"], "fallback_seconds": tm["fallback_seconds"],
-        "speedup_vs_shooting": shooting["gpu_seconds"] / secs,
-        "reference": "pB of the shooting leg (1024 replicas per input)",
-        # reliability of the network alone on these inputs: mean shooting pB per bin of prediction
-        "calibration": SG.calibration(tm["pred"][ok], ref[ok], bins=5),
-        "training": {k: float(blob[k]) for k in ("val_mae", "n_train") if k in blob.files},
-    }
+    band = ok & (ref >= 0.2) & (ref <= 0.8)
+
+    def accuracy(est):
+        err = np.abs(est - ref)
+        return {"mae_vs_shooting": float(err[ok].mean()),
+                "mae_band_0.2_0.8": float(err[band].mean()) if band.any() else None,
+                "max_error": float(err[ok].max())}
+
+    plain = {}
+    for reps in (16, 32, 64, 128, 256, 512):
+        t0 = time.perf_counter()
+        pb, _, _, _ = SG.label_by_shooting(inputs, reps=reps, seed=5050 + reps)
+        secs = time.perf_counter() - t0
+        plain[str(reps)] = {"seconds": secs, "value": n_inputs / secs, **accuracy(pb)}
+    out = {"metric": "committor pB estimates/sec (surrogate, shooting fallback for pB in [0.2, 0.8])",
+           "unit": "pB estimates/s", "inputs": n_inputs, "band_inputs": int(band.sum()),
+           "reference": "pB of the shooting leg (1024 replicas per input, the same 512 inputs)",
+           "plain_shooting": plain}
+    for name, path in (("mixed", SURROGATE_PARAMS), ("config5", SURROGATE_PARAMS_TRAJ)):
+        if not os.path.exists(path):
+            continue
+        blob = np.load(path)
+        t0 = time.perf_counter()
+        est, unsure, tm = SG.estimate(inputs, blob["params"], reps=fallback_reps, seed=4040)
+        secs = time.perf_counter() - t0
+        acc = accuracy(est)
+        # plain shooting at the smallest replica count whose MAE is at most the surrogate path's
+        match = next((k for k in plain if plain[k]["mae_vs_shooting"] <= acc["mae_vs_shooting"]), None)
+        cost = _training_cost(path, blob)
+        train_s = cost.get("label_seconds", 0.0) + cost.get("train_seconds", 0.0)
+        res = {"value": n_inputs / secs, "seconds": secs, **acc, "fallback_fraction": float(unsure.mean()),
+               "fallback_replicas": fallback_reps, "predict_seconds": tm["predict_seconds"],
+               "fallback_seconds": tm["fallback_seconds"],
+               "network_only": accuracy(tm["pred"]),
+               "calibration": SG.calibration(tm["pred"][ok], ref[ok], bins=5),
+               "training": cost,
+               "plain_shooting_same_mae": match,
+               "gain_vs_plain_same_mae": plain[match]["seconds"] / secs if match else None,
+               "gain_vs_plain_fallback_replicas": plain[str(fallback_reps)]["seconds"] / secs,
+               "training_gpu_seconds_per_estimate_this_run": train_s / n_inputs}
+        if match and plain[match]["seconds"] > secs:
+            res["break_even_estimates"] = train_s / ((plain[match]["seconds"] - secs) / n_inputs)
+        out[name] = res
+    if "mixed" in out:
+        out["value"] = out["mixed"]["value"]
+    return out
 
 
 def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
@@ -232,18 +282,51 @@ def _port_baseline(ncores: int, max_seconds: float) -> dict:
                       f"per-replica MT19937 (src/mc_cpu.c arithmetic restated in oracle/), {dt:.1f} s"}
 
 
+def host_description() -> dict:
+    """The host cores this process may use: nproc, the affinity set, the
+    cgroup CPU quota and lscpu's model line."""
+    import subprocess
+
+    out = {"nproc": os.cpu_count()}
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        out["affinity"] = os.cpu_count()
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        out["cgroup_cpus"] = None if q == "max" else float(q) / float(per)
+    except (OSError, ValueError):
+        out["cgroup_cpus"] = None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        info = dict(line.split(":", 1) for line in txt.splitlines() if ":" in line)
+        out["model"] = " ".join(info.get("Model name", "").split())
+        out["cpu_family"] = info.get("CPU family", "").strip()
+        out["cpu_model"] = info.get("Model", "").strip()
+        out["threads_per_core"] = info.get("Thread(s) per core", "").strip()
+    except Exception:
+        pass
+    return out
+
+
+def host_cores(host: dict) -> int:
+    """Every core of the affinity set, bounded by the cgroup quota."""
+    n = host.get("affinity") or 1
+    if host.get("cgroup_cpus"):
+        n = min(n, max(1, int(host["cgroup_cpus"])))
+    return max(1, int(n))
+
+
 def cpu_baseline(max_seconds: float = 12.0) -> dict:
-    """The reference's own mc_cpu.c (compiled from its sources into oracle/_ref) on all allowed host
-    cores (at most 16), one process per core (its MT19937 state is global), run in a child process
+    """The reference's own mc_cpu.c (compiled from its sources into oracle/_ref) on every host core
+    this process may use, one process per core (its MT19937 state is global), run in a child process
     that does not use the GPU; the oracle's threaded restatement is reported beside it and stands
     in when oracle/_ref was not built."""
     import subprocess
 
-    try:
-        ncores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(ncores, 16))
+    host = host_description()
+    ncores = host_cores(host)
     port = _port_baseline(ncores, max_seconds / 2)
     try:
         r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--cores", str(ncores), "--seconds",
@@ -253,8 +336,10 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
     except Exception:
         ref = None
     if ref is None:
+        port["host"] = host
         return port
     ref["port"] = port
+    ref["host"] = host
     try:
         r1 = subprocess.run([sys.executable, "-m", "oracle.refbench", "--cores", "1", "--seconds", "4", "--L",
                              str(L)], cwd=ROOT, capture_output=True, text=True, timeout=120)
@@ -268,6 +353,79 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
     except Exception:
         pass
     return ref
+
+
+def cpu_committor(ncores: int, replicas: int = 64) -> dict | None:
+    """The reference's CPU committor, src/mc_cpu.c mc_driver_cpu with itask 1 (compiled into
+    oracle/_ref), run to completion on committor inputs 0 .. ncores-1 with `replicas` replicas each,
+    one process per core, in a child process that does not use the GPU."""
+    import subprocess
+
+    try:
+        r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--committor", "--cores", str(ncores),
+                            "--replicas", str(replicas)], cwd=ROOT, capture_output=True, text=True, timeout=600)
+        return json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except Exception:
+        return None
+
+
+def throughput_run(G, method: int, steps: int, warmup: int, dump: bool):
+    """The multispin throughput workload with `method`'s kernel: `warmup` untimed
+    steps, a restart from all -1, then `steps` timed steps of one 100-sweep
+    launch with the fused check and the host reading the count, a bit-packed
+    gridstates.bin record to host memory every 10th step.  Returns the wall
+    time, the sweep kernel's time (HIP events) and the nucleated fractions."""
+    rng = G.RngState(R, 20251202)
+    eng = G.Engine(L, R, method, rng, BETA, H)
+    g0 = -np.ones((R, L, L), dtype=np.int32)
+    buf = np.empty((R * L * L + 7) // 8, dtype=np.uint8)  # gridstates.bin record body
+    eng.set_grids(g0)
+    for _ in range(warmup):
+        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
+        eng.counts()
+    eng.set_grids(g0)
+    eng.sync()
+    ms0 = eng.kernel_ms()
+    frac = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
+        c = eng.counts()  # synchronises
+        frac.append(c[0] / R)
+        if dump and (step + 1) % DUMP_EVERY_STEPS == 0:
+            eng.gridstates(buf)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    kern_ms = eng.kernel_ms() - ms0
+    eng.close()
+    rng.free()
+    return wall, kern_ms, frac
+
+
+def sliced_leg(G, steps: int, warmup: int) -> dict:
+    """The same workload with the bit-sliced multispin kernel (engine gpu_method
+    3: the 64 replicas of a word share one site per attempt; neighbour sums by
+    bitwise adders, acceptance by bit-sliced comparison of per-replica draws),
+    at the tuner's launch shape."""
+    S3, split3 = G.get_tuning(3, L, R)
+    wall, kern_ms, frac = throughput_run(G, 3, steps, warmup, True)
+    attempts = float(steps) * SWEEPS_PER_STEP * L * L * R
+    per_launch_ms = kern_ms / max(steps, 1)
+    wb = 64 // split3
+    sweeps = np.arange(1, steps + 1) * SWEEPS_PER_STEP
+    k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
+    # LDS: five 4- or 8-byte word reads per attempt serve the wb replicas of the word
+    bpa = 5.0 * (wb // 8) / wb
+    achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
+    return {"metric": "spin-flip attempts/sec, bit-sliced multispin (shared site per word)",
+            "value": attempts / wall, "unit": "attempts/s", "ms_per_step": wall * 1e3 / max(steps, 1),
+            "steps": steps, "kernel": f"sweep_sliced<{wb}, {S3}, true>", "waves_per_workgroup": S3,
+            "replicas_per_word": wb, "kernel_ms_per_launch": per_launch_ms,
+            "kernel_attempts_per_s": attempts / (kern_ms / 1e3),
+            "lds": {"bytes_per_attempt": bpa, "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / LDS_PEAK_GBS},
+            "nucleation_fit": {"k_per_sweep": k, "t0": t0fit, "final_fraction": frac[-1] if frac else None,
+                               "reference_k_per_sweep": 1.5032e-4}}
 
 
 def main() -> int:
@@ -285,6 +443,7 @@ def main() -> int:
     ap.add_argument("--no-committor", action="store_true", help="skip the committor shooting leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (graph) leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate committor leg")
+    ap.add_argument("--no-sliced", action="store_true", help="skip the bit-sliced multispin leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas are independent "
@@ -295,34 +454,8 @@ def main() -> int:
 
     G.init_device(-1)
     G.set_tuning(args.slots, args.word_split)
-    rng = G.RngState(R, 20251202)
-    eng = G.Engine(L, R, args.method, rng, BETA, H)
-    g0 = -np.ones((R, L, L), dtype=np.int32)
-    dump = np.empty((R * L * L + 7) // 8, dtype=np.uint8)  # gridstates.bin record body
     S, split = G.get_tuning(args.method, L, R)
-
-    # warmup (untimed), then restart from all -1
-    eng.set_grids(g0)
-    for _ in range(args.warmup):
-        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
-        eng.counts()
-    eng.set_grids(g0)
-    eng.sync()
-
-    ms0 = eng.kernel_ms()
-    frac = []
-    t0 = time.perf_counter()
-    for step in range(args.steps):
-        eng.sweep(SWEEPS_PER_STEP, G.CHECK_COUNT, DN_THR, UP_THR)
-        c = eng.counts()  # synchronises
-        frac.append(c[0] / R)
-        if not args.no_dump and (step + 1) % DUMP_EVERY_STEPS == 0:
-            eng.gridstates(dump)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    kern_ms = eng.kernel_ms() - ms0
-    eng.close()
-    rng.free()
+    wall, kern_ms, frac = throughput_run(G, args.method, args.steps, args.warmup, not args.no_dump)
 
     attempts = float(args.steps) * SWEEPS_PER_STEP * L * L * R
     value = attempts / wall
@@ -335,7 +468,7 @@ def main() -> int:
     peak = HBM_PEAK_GBS if hbm else LDS_PEAK_GBS
     achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
     multi = {1: "LAT_MULTISPIN", 2: "LAT_MULTIHALF", 4: "LAT_MULTI16", 8: "LAT_MULTI8"}
-    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: multi.get(split, "LAT_MULTISPIN")}[args.method]
+    kname = {0: "LAT_HBM", 1: "LAT_BITS", 2: multi.get(split, "LAT_MULTISPIN"), 3: "sliced"}[args.method]
     sweeps = np.arange(1, args.steps + 1) * SWEEPS_PER_STEP
     k, t0fit = fit_rate(sweeps.astype(float), np.array(frac))
 
@@ -382,8 +515,17 @@ def main() -> int:
     if args.method == 2 and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             prof = json.load(f)
-        out["roofline"]["traffic"] = prof.get("hbm_bytes_per_launch")
-        vpa = prof.get("valu_instructions_per_attempt")
+        # the counters hold for the kernel source they were measured on
+        import hashlib
+
+        with open(os.path.join(ROOT, "gpu_2dising_amd", "csrc", "gasp_kernels.h"), "rb") as f:
+            src_sha = hashlib.sha256(f.read()).hexdigest()
+        current = prof.get("kernel_source_sha256") == src_sha
+        out["roofline"]["traffic"] = prof.get("hbm_bytes_per_launch") if current else None
+        vpa = prof.get("valu_instructions_per_attempt") if current else None
+        out["roofline"]["counters"] = {"file": "profiles/traffic_sweep_multispin_L64.json",
+                                       "kernel_source_sha256": prof.get("kernel_source_sha256"),
+                                       "matches_current_source": current}
         if vpa:
             # VALU issue: wave64 instructions per attempt (rocprofv3 SQ_INSTS_VALU over the same
             # launch) x the live attempt rate, against 256 CU x 4 SIMD x 2.4 GHz / 4 cycles
@@ -393,11 +535,12 @@ def main() -> int:
             # the binding resource is the higher of the two fractions
             out["roofline"]["limiter"] = "valu" if kern_rate * vpa / VALU_PEAK > achieved / peak else "lds"
     G.set_tuning(0, 0)
+    if not args.no_sliced and args.method != 3:
+        out["sliced"] = sliced_leg(G, args.steps, args.warmup)
     if not args.no_committor:
         out["committor"] = committor_leg(G)
-        if os.path.exists(SURROGATE_PARAMS) and not args.no_surrogate:
+        if not args.no_surrogate and (os.path.exists(SURROGATE_PARAMS) or os.path.exists(SURROGATE_PARAMS_TRAJ)):
             out["surrogate"] = surrogate_leg(G, out["committor"])
-        out["committor"].pop("_pB")
     if not args.no_batched:
         out["batched"] = batched_leg(G)
     if not args.no_cpu:
@@ -405,14 +548,20 @@ def main() -> int:
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
         if "committor" in out:
-            # The reference's CPU committor (src/mc_cpu.c:190-267) sweeps every replica of every
-            # input until all fates are resolved, so one pB estimate costs sweeps x replicas x N
-            # attempts at the CPU baseline's attempt rate (same mc_sweep_cpu arithmetic).
+            # The reference's CPU committor (src/mc_cpu.c:190-267: every replica of the input swept
+            # until all its fates are decided), measured to completion on one input per host core at
+            # 64 replicas per input.  An estimate at the shooting leg's 1024 replicas sweeps 16 times
+            # the replicas for at least as many sweeps, so its rate is at most 64/1024 of that.
             cm = out["committor"]
-            ape = float(cm["sweeps"]) * cm["replicas_per_input"] * L * L
-            cm["cpu_value"] = cb["value"] / ape
-            cm["cpu_basis"] = (f"cpu_baseline attempts/s / {ape:.4g} attempts per estimate "
-                               f"({cm['sweeps']} sweeps x {cm['replicas_per_input']} replicas x {L * L} sites)")
+            cc = cpu_committor(host_cores(cb.get("host", {})))
+            if cc is not None:
+                ref_pB = cm["_pB"][cc["inputs"]]
+                cc["mae_vs_gpu_shooting_1024"] = float(np.nanmean(np.abs(np.array(cc["pB"]) - ref_pB)))
+                cc["value_1024_upper_bound"] = cc["value"] * cc["replicas_per_input"] / cm["replicas_per_input"]
+                cm["cpu"] = cc
+                cm["gpu_over_cpu_1024"] = cm["value"] / cc["value_1024_upper_bound"]
+    if "committor" in out:
+        out["committor"].pop("_pB")
     print(json.dumps(out), flush=True)
     return 0
 

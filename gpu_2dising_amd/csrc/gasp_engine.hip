@@ -355,7 +355,10 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
 
 // Method 3: waves per workgroup (*S_out: 1, 2, 4, 8) and word split
 // (*W_out: 1 = 64 replicas per word, 2 = 32).  Default: whole words, and
-// waves per workgroup until there are about four waves per CU.
+// waves per workgroup until there are about eight waves per CU: a round
+// waits on LDS round trips and two barriers, which one wave per SIMD does not
+// hide (L = 64, 16384 replicas, 100-sweep launches: 16.5 ms with one wave
+// per workgroup, 9.5 with two, 6.2 with four, 4.8 with eight).
 static void choose_sliced(int L, int R, int *S_out, int *W_out) {
     if (g_tune_slots <= 0 && g_tune_waves <= 0) {
         auto it = g_tuned.find(std::make_tuple(3, L, R));
@@ -373,7 +376,7 @@ static void choose_sliced(int L, int R, int *S_out, int *W_out) {
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), 8);
     } else {
-        while (S < 8 && groups * S < 4LL * ncu) S *= 2;
+        while (S < 8 && groups * S < 8LL * ncu) S *= 2;
     }
     *S_out = S;
     *W_out = split;

@@ -254,6 +254,7 @@ def main():
     ap.add_argument("--reps", type=int, default=256)
     ap.add_argument("--sweeps", type=int, default=50000)
     ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--name", default="surrogate_L64", help="file name of the parameters and summary")
     ap.add_argument("--bench", action="store_true",
                     help="only time inference and training with the packaged parameters")
     args = ap.parse_args()
@@ -295,9 +296,10 @@ def main():
                "calibration": calibration(pv, yv),
                "loss_history": [float(x) for x in met["loss_history"]]}
     np.save(os.path.join(args.out, "val_idx.npy"), val)
-    np.savez(os.path.join(args.out, "surrogate_L64.npz"), params=params, val_mae=summary["val_mae"],
-             n_train=len(g) - len(val), epochs=args.epochs, reps=args.reps)
-    with open(os.path.join(args.out, "summary.json"), "w") as f:
+    np.savez(os.path.join(args.out, args.name + ".npz"), params=params, val_mae=summary["val_mae"],
+             n_train=len(g) - len(val), epochs=args.epochs, reps=args.reps, label_seconds=ts,
+             train_seconds=met["train_seconds"], n_trajectory=len(idx), n_discs=args.discs)
+    with open(os.path.join(args.out, args.name + ".json"), "w") as f:
         json.dump(summary, f, indent=1)
     print(json.dumps({k: v for k, v in summary.items() if k != "loss_history"}), flush=True)
 

@@ -72,13 +72,9 @@ def run(cores: int, seconds: float, L: int = 64, beta: float = 0.54, h: float = 
                       f"{nsweeps} sweeps, L={L}, beta={beta}, h={h}, all -1 start, {dt:.1f} s"}
 
 
-def _driver_child(nsweeps, q):
+def _driver_types(R):
+    """ctypes signature of src/mc_cpu.c mc_driver_cpu on the compiled reference."""
     import ctypes as C
-    from oracle import oracle as O
-
-    # the driver prints a line per check: keep it off this process's stdout
-    devnull = os.open(os.devnull, os.O_WRONLY)
-    os.dup2(devnull, 1)
 
     class McGrids(C.Structure):  # include/mc_cpu.h:5-9
         _fields_ = [("L", C.c_int), ("ngrids", C.c_int), ("ising_grids", C.POINTER(C.c_int))]
@@ -93,9 +89,24 @@ def _driver_child(nsweeps, q):
 
     outf_t = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_float),
                          C.POINTER(C.c_float), C.c_char_p, C.c_double, C.c_double, C.c_char_p)
-    R = O.ref()
     R.mc_driver_cpu.argtypes = [McGrids, C.c_double, C.c_double, C.POINTER(C.c_int), McSampler, McFunction, outf_t]
     R.mc_driver_cpu.restype = None
+    return McGrids, McSampler, McFunction, outf_t
+
+
+def _quiet_stdout():
+    # the driver prints a line per check: keep it off this process's stdout
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+
+
+def _driver_child(nsweeps, q):
+    import ctypes as C
+    from oracle import oracle as O
+
+    _quiet_stdout()
+    R = O.ref()
+    McGrids, McSampler, McFunction, outf_t = _driver_types(R)
     L, n, beta, h, mag = 32, 256, 0.54, 0.07, 100
     grids = -np.ones(n * L * L, dtype=np.int32)
     fates = np.full(n, -1, dtype=np.int32)
@@ -114,6 +125,64 @@ def _driver_child(nsweeps, q):
     full = np.nonzero(result[: nsweeps // mag + 1] >= 1.0)[0]
     done = int(full[0]) * mag if full.size else nsweeps
     q.put((done * n * L * L, dt))
+
+
+def _committor_child(i, grid, replicas, max_sweeps, seed, start, q):
+    """src/mc_cpu.c:106-308 with itask 1 on one input grid: `replicas` copies
+    (src/grid.c:41-71), one MT19937 stream seeded seed + i over all of them,
+    checks every 100 sweeps against -0.90 / +0.90, every replica swept until
+    every fate is decided or max_sweeps."""
+    import ctypes as C
+    from oracle import oracle as O
+
+    _quiet_stdout()
+    R = O.ref()
+    McGrids, McSampler, McFunction, outf_t = _driver_types(R)
+    L = grid.shape[-1]
+    beta, h, mag = 0.54, 0.07, 100
+    grids = np.ascontiguousarray(np.broadcast_to(grid.reshape(1, -1), (replicas, L * L)), dtype=np.int32)
+    fates = np.full(replicas, -1, dtype=np.int32)
+    result = np.zeros(1, dtype=np.float32)
+    noop = outf_t(lambda *args: 0)
+    R.init_genrand(seed + i)
+    R.preComputeProbs_cpu(beta, h)
+    start.wait()
+    t0 = time.perf_counter()
+    R.mc_driver_cpu(McGrids(L, replicas, grids.ctypes.data_as(C.POINTER(C.c_int))), beta, h,
+                    fates.ctypes.data_as(C.POINTER(C.c_int)), McSampler(max_sweeps, mag, 100 * max_sweeps),
+                    McFunction(1, -0.9, 0.9, 1, -1, b"magnetisation", result.ctypes.data_as(C.POINTER(C.c_float)),
+                               b"gridstates.bin"), noop)
+    q.put((i, float(result[0]), int((fates == -1).sum()), time.perf_counter() - t0))
+
+
+def run_committor(cores: int, replicas: int = 64, max_sweeps: int = 20000, seed: int = 31) -> dict:
+    """The reference's CPU committor (mc_driver_cpu, itask 1) to completion on
+    a bounded sample: inputs 0 .. cores-1 of the committor workload (512
+    droplet grids, L = 64), one per process, `replicas` replicas each; every
+    process starts together and the rate is inputs over the slowest process's
+    time."""
+    from gpu_2dising_amd.surrogate import disc_grids
+    from oracle import oracle as O
+
+    if O.ref() is None:
+        raise RuntimeError("oracle/_ref/libgasp_ref.so is not built")
+    inputs = disc_grids(512, 64, 12345)[:cores]
+    ctx = mp.get_context("fork")
+    start, q = ctx.Barrier(cores), ctx.Queue()
+    procs = [ctx.Process(target=_committor_child, args=(i, inputs[i], replicas, max_sweeps, seed, start, q))
+             for i in range(cores)]
+    for p in procs:
+        p.start()
+    res = sorted(q.get() for _ in procs)
+    for p in procs:
+        p.join()
+    dt = max(r[3] for r in res)
+    return {"value": cores / dt, "unit": "pB estimates/s", "cores": cores, "kind": "reference",
+            "replicas_per_input": replicas, "inputs": list(range(cores)), "pB": [r[1] for r in res],
+            "unresolved": sum(r[2] for r in res), "seconds": dt,
+            "sample": f"src/mc_cpu.c mc_driver_cpu itask 1 (compiled reference, one MT19937 stream per process), "
+                      f"committor inputs 0..{cores - 1} (L=64 droplets), {replicas} replicas each, checks every 100 "
+                      f"sweeps, thresholds -0.90/+0.90, run to completion, {cores} processes, {dt:.1f} s"}
 
 
 def run_driver(nsweeps: int = 1000) -> dict:
@@ -141,8 +210,15 @@ def main() -> int:
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--driver", action="store_true", help="time mc_driver_cpu on config 1's shape")
     ap.add_argument("--sweeps", type=int, default=1000, help="--driver: sweeps")
+    ap.add_argument("--committor", action="store_true",
+                    help="time mc_driver_cpu itask 1 on committor inputs 0..cores-1")
+    ap.add_argument("--replicas", type=int, default=64, help="--committor: replicas per input")
     a = ap.parse_args()
-    print(json.dumps(run_driver(a.sweeps) if a.driver else run(a.cores, a.seconds, a.L)), flush=True)
+    if a.committor:
+        out = run_committor(a.cores, a.replicas)
+    else:
+        out = run_driver(a.sweeps) if a.driver else run(a.cores, a.seconds, a.L)
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -170,3 +170,89 @@ def test_sliced_caller_grids(gpu, engine_lib, oracle):
     rng.free()
     hip.free(d_g)
     hip.free(d_m)
+
+
+# ---------------------------------------------------------------- physics
+# The sliced mode's chains are random-site Metropolis per replica, so the
+# reference's published physics must come out within its standard errors;
+# the replicas of a word share their sites, so errors are estimated with
+# words (not replicas) as the independent units.
+def _first_passage(mags_per_check, up):
+    """index of the first check with m > up per replica (n_checks if never)"""
+    m = np.asarray(mags_per_check)
+    hit = m > up
+    return np.where(hit.any(0), hit.argmax(0), m.shape[0])
+
+
+def test_sliced_nucleation_rate_mode_and_word_correlation(gpu, engine_lib):
+    from scipy.optimize import curve_fit
+
+    G = engine_lib
+    L, R, steps, spc = 64, 16384, 500, 100
+    print(f"[sliced physics] L={L} R={R} {steps * spc} sweeps", flush=True)
+    rng = G.RngState(R, 4711)
+    frac, mags = [], []
+    hist = np.zeros(50)
+    edges = np.linspace(-1.0, -0.8, 51)
+    with G.Engine(L, R, 3, rng, BETA, H) as e:
+        e.set_grids(-np.ones((R, L, L), dtype=np.int32))
+        for s in range(steps):
+            e.sweep(spc, G.CHECK_COUNT, -0.9, 0.9)
+            frac.append(e.counts()[0] / R)
+            m = e.magnetisation()
+            mags.append(m)
+            if s < 200:
+                hist += np.histogram(m, bins=edges)[0]
+    rng.free()
+    mode = edges[np.argmax(hist)]
+    assert abs(mode - (-0.94)) <= 0.0041 + 1e-12, mode
+    t = np.arange(1, steps + 1) * spc
+    f = np.array(frac)
+    k, t0 = curve_fit(lambda t, k, t0: 1 - np.exp(-k * (t - t0)), t[f > 0], f[f > 0], p0=(1.5e-4, 500))[0]
+    fp = _first_passage(mags, 0.9).astype(float)
+    w = fp.reshape(-1, 64)
+    # correlation of first-passage checks within a word against between words
+    c = np.corrcoef(w.T)
+    within = c[np.triu_indices(64, 1)].mean()
+    between = np.corrcoef(w[:, :32].reshape(-1), np.roll(w[:, 32:], 1, axis=0).reshape(-1))[0, 1]
+    print(f"[sliced physics] k = {k:.5e} (reference 1.5032e-4), mode {mode:.3f}, first-passage correlation "
+          f"within a word {within:.4f}, between words {between:.4f}", flush=True)
+    assert abs(k / 1.5032e-4 - 1.0) < 0.08
+    assert within < 0.2
+
+
+def _square(L=64, size=6):
+    g = np.ones((L, L), dtype=np.int32)
+    r = np.arange(L)
+    out = (np.abs(r[:, None] - L / 2) > size) | (np.abs(r[None, :] - L / 2) > size)
+    g[out] = -1
+    return g
+
+
+def test_sliced_square_droplet_committor(gpu, engine_lib):
+    """pB of the 13 x 13 square droplet (thresholds -0.93 / 0.90), 10240
+    replicas (160 words): 0.78184 +- 0.00597 in the reference's notebook; the
+    standard error here is that of the 160 word means."""
+    G = engine_lib
+    L, R = 64, 10240
+    print(f"[sliced physics] square droplet, {R} replicas", flush=True)
+    rng = G.RngState(R, 8080)
+    with G.Engine(L, R, 3, rng, BETA, H) as e:
+        e.set_inputs(_square()[None])
+        e.set_freeze(True)
+        e.set_fates(np.full(R, -1, dtype=np.int32))
+        e.sweep(0, G.CHECK_FATES, -0.93, 0.90)
+        sweeps, c = 0, e.counts()
+        while sweeps < 20000 and c[1] + c[2] < R:
+            e.sweep(100, G.CHECK_FATES, -0.93, 0.90)
+            sweeps += 100
+            c = e.counts()
+        fates = e.fates()
+    rng.free()
+    assert (fates >= 0).all()
+    pB = (fates == 1).mean()
+    wm = (fates.reshape(-1, 64) == 1).mean(1)
+    se = wm.std(ddof=1) / np.sqrt(wm.size)
+    binom = np.sqrt(pB * (1 - pB) / R)
+    print(f"[sliced physics] pB = {pB:.4f} +- {se:.4f} (binomial {binom:.4f}), {sweeps} sweeps", flush=True)
+    assert abs(pB - 0.78184) < 3.5 * np.hypot(se, 0.00597)
