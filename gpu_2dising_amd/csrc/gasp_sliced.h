@@ -97,6 +97,43 @@ constexpr int kAndAndNot = 0x40;   // A & B & ~C
 constexpr int kAndXnor = 0x90;     // A & ~(B ^ C)
 constexpr int kNor3 = 0x01;        // ~(A | B | C)
 
+// a & m and (a & m) | c for a wave-uniform mask m (all ones or zero) held in
+// a scalar register: one v_and_b32 / v_and_or_b32.  Written out because the
+// compiler otherwise turns the known all-or-nothing mask into a select.
+__device__ __forceinline__ uint32_t and_s(uint32_t a, uint32_t m) {
+    uint32_t r;
+    asm("v_and_b32 %0, %1, %2" : "=v"(r) : "s"(m), "v"(a));
+    return r;
+}
+__device__ __forceinline__ uint32_t and_or_s(uint32_t a, uint32_t m, uint32_t c) {
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(c));
+    return r;
+}
+
+// Two Philox4x32-10 blocks with the same key, their rounds interleaved.
+__device__ __forceinline__ void philox4x32_10x2(uint32_t (&x)[4], uint32_t (&y)[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint64_t px0 = (uint64_t)kPhiloxM0 * x[0], px1 = (uint64_t)kPhiloxM1 * x[2];
+        const uint64_t py0 = (uint64_t)kPhiloxM0 * y[0], py1 = (uint64_t)kPhiloxM1 * y[2];
+        const uint32_t xn0 = __builtin_amdgcn_bitop3_b32((uint32_t)(px1 >> 32), x[1], k0, kXor3);
+        const uint32_t xn2 = __builtin_amdgcn_bitop3_b32((uint32_t)(px0 >> 32), x[3], k1, kXor3);
+        const uint32_t yn0 = __builtin_amdgcn_bitop3_b32((uint32_t)(py1 >> 32), y[1], k0, kXor3);
+        const uint32_t yn2 = __builtin_amdgcn_bitop3_b32((uint32_t)(py0 >> 32), y[3], k1, kXor3);
+        x[1] = (uint32_t)px1;
+        x[3] = (uint32_t)px0;
+        x[0] = xn0;
+        x[2] = xn2;
+        y[1] = (uint32_t)py1;
+        y[3] = (uint32_t)py0;
+        y[0] = yn0;
+        y[2] = yn2;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+}
+
 template <int WB, int W, bool POW2>
 __global__ void __launch_bounds__(64 * W) sweep_sliced(SweepArgs a) {
     static_assert(WB == 64 || WB == 32, "64 or 32 replicas per word");
@@ -167,22 +204,35 @@ __global__ void __launch_bounds__(64 * W) sweep_sliced(SweepArgs a) {
             const unsigned base = kSlLat + (unsigned)st.c;
             __hip_atomic_fetch_min(lds_ptr32(base + 8u + 4u * par), l, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            uint32_t up[Q], dn[Q], lt[Q], rt[Q];
+            uint64_t c64, u64, d64, l64, r64;
             if constexpr (Q == 2) {
-                const uint64_t c64 = lds_load64(base), u64 = lds_load64(kSlLat + (unsigned)st.up),
-                               d64 = lds_load64(kSlLat + (unsigned)st.dn), l64 = lds_load64(kSlLat + (unsigned)st.lt),
-                               r64 = lds_load64(kSlLat + (unsigned)st.rt);
-                cw[0] = (uint32_t)c64; cw[1] = (uint32_t)(c64 >> 32);
-                up[0] = (uint32_t)u64; up[1] = (uint32_t)(u64 >> 32);
-                dn[0] = (uint32_t)d64; dn[1] = (uint32_t)(d64 >> 32);
-                lt[0] = (uint32_t)l64; lt[1] = (uint32_t)(l64 >> 32);
-                rt[0] = (uint32_t)r64; rt[1] = (uint32_t)(r64 >> 32);
+                c64 = lds_load64(base);
+                u64 = lds_load64(kSlLat + (unsigned)st.up);
+                d64 = lds_load64(kSlLat + (unsigned)st.dn);
+                l64 = lds_load64(kSlLat + (unsigned)st.lt);
+                r64 = lds_load64(kSlLat + (unsigned)st.rt);
             } else {
-                cw[0] = lds_load32(base);
-                up[0] = lds_load32(kSlLat + (unsigned)st.up);
-                dn[0] = lds_load32(kSlLat + (unsigned)st.dn);
-                lt[0] = lds_load32(kSlLat + (unsigned)st.lt);
-                rt[0] = lds_load32(kSlLat + (unsigned)st.rt);
+                c64 = lds_load32(base);
+                u64 = lds_load32(kSlLat + (unsigned)st.up);
+                d64 = lds_load32(kSlLat + (unsigned)st.dn);
+                l64 = lds_load32(kSlLat + (unsigned)st.lt);
+                r64 = lds_load32(kSlLat + (unsigned)st.rt);
+            }
+            // Blocks 1 and 2 hold planes that nearly every round needs (planes
+            // 1-4 of whole words, 3-10 of half words); they do not depend on
+            // the lattice, so they are computed while its reads are in flight
+            // (two independent chains side by side).
+            uint32_t v1[4] = {(uint32_t)A, (uint32_t)(A >> 32), (uint32_t)g, kSlicedDomain + 1u};
+            uint32_t v2[4] = {(uint32_t)A, (uint32_t)(A >> 32), (uint32_t)g, kSlicedDomain + 2u};
+            philox4x32_10x2(v1, v2, k0, k1);
+            uint32_t up[Q], dn[Q], lt[Q], rt[Q];
+#pragma unroll
+            for (int d = 0; d < Q; ++d) {
+                cw[d] = (uint32_t)(c64 >> (32 * d));
+                up[d] = (uint32_t)(u64 >> (32 * d));
+                dn[d] = (uint32_t)(d64 >> (32 * d));
+                lt[d] = (uint32_t)(l64 >> (32 * d));
+                rt[d] = (uint32_t)(r64 >> (32 * d));
             }
             // slot masks: the replicas whose class is slot i's
             uint32_t ms[5][Q];
@@ -198,51 +248,85 @@ __global__ void __launch_bounds__(64 * W) sweep_sliced(SweepArgs a) {
 #pragma unroll
                 for (int i = 0; i < 5; ++i) {
                     // all-ones where the bit of the code is set (uniform)
-                    const uint32_t v0 = 0u - (sl.code[i] & 1u), v1 = 0u - ((sl.code[i] >> 1) & 1u),
-                                   v2 = 0u - ((sl.code[i] >> 2) & 1u), vs = 0u - ((sl.code[i] >> 3) & 1u);
-                    const uint32_t eq = __builtin_amdgcn_bitop3_b32(b0 ^ v0, b1 ^ v1, b2 ^ v2, kNor3);
+                    const uint32_t v0 = 0u - (sl.code[i] & 1u), vb1 = 0u - ((sl.code[i] >> 1) & 1u),
+                                   vb2 = 0u - ((sl.code[i] >> 2) & 1u), vs = 0u - ((sl.code[i] >> 3) & 1u);
+                    const uint32_t eq = __builtin_amdgcn_bitop3_b32(b0 ^ v0, b1 ^ vb1, b2 ^ vb2, kNor3);
                     ms[i][d] = eq & ~(cw[d] ^ vs);
                     any |= ms[i][d];
                 }
                 und[d] = any & allow[d];
                 acc[d] = ~any & allow[d];  // classes at threshold 2^32 always flip
             }
-            // plane 0 is in block 0
+            // Plane p: Tj = the replicas whose class threshold has bit 31 - p
+            // set, the union of the slot masks selected by uniform all-ones /
+            // zero masks (one v_and_or_b32 per slot and dword); then a replica
+            // still undecided flips if Tj is set where the draw's bit is not,
+            // and stays undecided where the two bits agree.
+            // The planes are applied in order, so the thresholds are carried
+            // shifted left by the plane index: bit 31 holds plane p's bit, and
+            // an arithmetic shift by 31 gives the mask (scalar operations).
+            uint32_t tsh[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) tsh[i] = sl.thr[i];
             auto plane = [&](int p, const uint32_t *rnd) {
+                (void)p;
+                uint32_t sel[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    sel[i] = (uint32_t)((int32_t)tsh[i] >> 31);
+                    tsh[i] <<= 1;
+                }
 #pragma unroll
                 for (int d = 0; d < Q; ++d) {
-                    uint32_t tj = 0u;
+                    uint32_t tj = and_s(ms[0][d], sel[0]);
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) tj |= ms[i][d] & (0u - ((sl.thr[i] >> (31 - p)) & 1u));
+                    for (int i = 1; i < 5; ++i) tj = and_or_s(ms[i][d], sel[i], tj);
                     acc[d] |= __builtin_amdgcn_bitop3_b32(und[d], tj, rnd[d], kAndAndNot);
                     und[d] = __builtin_amdgcn_bitop3_b32(und[d], tj, rnd[d], kAndXnor);
                 }
             };
+            int p;
             if constexpr (Q == 2) {
                 plane(0, w + 2);
+                plane(1, v1);
+                plane(2, v1 + 2);
+                plane(3, v2);
+                plane(4, v2 + 2);
+                p = 5;
             } else {
                 plane(0, w + 1);
                 plane(1, w + 2);
                 plane(2, w + 3);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) plane(3 + k, v1 + k);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) plane(7 + k, v2 + k);
+                p = 11;
             }
-            // further blocks while any replica of the wave is undecided
-            int p = Q == 2 ? 1 : 3;
+            // two further blocks at a time while any replica of the wave is
+            // undecided (a block past the deciding one changes nothing)
             uint32_t pend = und[0];
 #pragma unroll
             for (int d = 1; d < Q; ++d) pend |= und[d];
+            blk = 2u;
             while (wave_mask(pend != 0u) != 0ull && p < 32) {
-                ++blk;
-                uint32_t v[4] = {(uint32_t)A, (uint32_t)(A >> 32), (uint32_t)g, kSlicedDomain + blk};
-                philox4x32_10(v[0], v[1], v[2], v[3], k0, k1);
+                uint32_t va[4] = {(uint32_t)A, (uint32_t)(A >> 32), (uint32_t)g, kSlicedDomain + blk + 1u};
+                uint32_t vb[4] = {(uint32_t)A, (uint32_t)(A >> 32), (uint32_t)g, kSlicedDomain + blk + 2u};
+                philox4x32_10x2(va, vb, k0, k1);
+                blk += 2u;
                 if constexpr (Q == 2) {
-                    plane(p, v);
-                    if (p + 1 < 32) plane(p + 1, v + 2);
-                    p += 2;
+                    plane(p, va);  // p <= 29 here
+                    plane(p + 1, va + 2);
+                    if (p + 2 < 32) plane(p + 2, vb);
+                    if (p + 3 < 32) plane(p + 3, vb + 2);
+                    p += 4;
                 } else {
 #pragma unroll
+                    for (int k = 0; k < 4; ++k) plane(p + k, va + k);
+#pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        if (p + k < 32) plane(p + k, v + k);
-                    p += 4;
+                        if (p + 4 + k < 32) plane(p + 4 + k, vb + k);
+                    p += 8;
                 }
                 pend = und[0];
 #pragma unroll

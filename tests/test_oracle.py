@@ -127,6 +127,57 @@ def test_pacc_is_metropolis_of_hamiltonian(oracle, beta, h):
             assert float(pf[idx]) == pytest.approx(np.exp(-beta * de), rel=1e-5)
 
 
+TABLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "acceptance_tables.json")
+
+
+def test_tables_match_hex_fixtures(oracle):
+    """Both acceptance tables and their integer thresholds, bit for bit, against
+    tests/golden/acceptance_tables.json (written by make_golden_tables.py with the
+    C library's expf / exp on the reference expressions) at the workloads' (beta, h)
+    and the batched configuration's five sampled ones."""
+    import json
+
+    with open(TABLES) as f:
+        fx = json.load(f)["tables"]
+    assert len(fx) >= 6
+    for t in fx:
+        pf = oracle.pacc_gpu(t["beta"], t["h"])
+        pd = oracle.pacc_cpu(t["beta"], t["h"])
+        assert [x.view(np.uint32).item().to_bytes(4, "big").hex() for x in pf] == t["f32_hex"], t["beta"]
+        assert [float(x).hex() for x in pd] == t["f64_hex"], t["beta"]
+        assert [int(x) for x in oracle.thresholds(pf.astype(np.float64))] == t["thr_f32"]
+        assert [int(x) for x in oracle.thresholds(pd)] == t["thr_f64"]
+
+
+@pytest.mark.parametrize("L", [3, 16, 20, 64])
+def test_production_site_map_uniform(oracle, L):
+    """The production Philox stream's sites as the oracle uses them, read back at
+    beta = 0 (every attempt flips its site): equal to floor(w N / 2^32) of an
+    independent Philox (words 0 / 2 of block (p_lo, p_hi, replica, "GASP") for
+    attempts 2p / 2p + 1) and uniform over the sites (chi-square)."""
+    from test_oracle_sliced import philox_np
+
+    N, seed, replica, n = L * L, 987654321987, 3, 6000
+    g = -np.ones((1, N), dtype=np.int32)
+    thr = oracle.thresholds(oracle.pacc_gpu(0.0, 0.0))
+    sites = np.empty(n, dtype=np.int64)
+    for i in range(n):
+        before = g[0].copy()
+        oracle.lib().orc_sweep_philox(L, g[0], seed, replica, 500 + i, 1, thr)
+        d = np.flatnonzero(g[0] != before)
+        assert d.size == 1
+        sites[i] = d[0]
+    a = 500 + np.arange(n, dtype=np.uint64)
+    p = a >> np.uint64(1)
+    w = philox_np(p & np.uint64(0xFFFFFFFF), p >> np.uint64(32), np.full(n, replica), np.full(n, 0x47415350),
+                  seed & 0xFFFFFFFF, seed >> 32)
+    site_word = np.where(a & np.uint64(1), w[2], w[0])
+    assert np.array_equal(sites, ((site_word * np.uint64(N)) >> np.uint64(32)).astype(np.int64))
+    from scipy import stats
+
+    assert stats.chisquare(np.bincount(sites, minlength=N)).pvalue > 1e-4
+
+
 # ------------------------------------------------------------------ sweeps
 @pytest.mark.parametrize("L", [4, 5, 8, 16, 17, 32, 48, 64])
 def test_sweep_matches_reference_build(oracle, gold, L):
@@ -157,69 +208,40 @@ def test_config1_matches_reference_build(oracle, gold):
 
 
 def test_boltzmann_3x3_exact_enumeration(oracle):
-    """Long Philox-stream chain on a 3x3 lattice: the visit histogram of the
-    (magnetisation, energy) classes matches the exact Boltzmann weights."""
-    L, beta, h = 3, 0.35, 0.2
-    states = np.array([[1 if (k >> i) & 1 else -1 for i in range(9)] for k in range(512)]).reshape(512, L, L)
+    """Philox-stream chains of 32 replicas on a 3x3 lattice visit the 512 states
+    with their exact Boltzmann weights.  Standard errors come from 40 batch means,
+    which measure the chains' autocorrelation (the integrated autocorrelation
+    time of the magnetisation is reported) instead of assuming a factor."""
+    from test_oracle_sliced import _boltzmann_3x3, _state_index
 
-    def energy(x):
-        return -np.sum(x * np.roll(x, 1, 0)) - np.sum(x * np.roll(x, 1, 1)) - h * np.sum(x)
-
-    E = np.array([energy(s) for s in states])
-    M = states.reshape(512, -1).sum(1)
-    w = np.exp(-beta * (E - E.min()))
-    keys = np.round(E * 1000).astype(np.int64) * 100 + M
-    uk, inv = np.unique(keys, return_inverse=True)
-    exact = np.bincount(inv, weights=w) / w.sum()
-
-    g = -np.ones((1, L, L), dtype=np.int32)
-    lib = oracle.lib()
-    nsamp, stride = 200000, 9
-    counts = np.zeros(len(uk))
-    flat = np.ascontiguousarray(g.reshape(1, -1))
-    att = 0
-    for _ in range(nsamp):
-        lib.orc_sweep_philox(L, flat[0], 77, 0, att, stride, oracle.thresholds(oracle.pacc_gpu(beta, h)))
-        att += stride
-        x = flat[0].reshape(L, L)
-        k = int(np.round(energy(x) * 1000)) * 100 + int(x.sum())
-        counts[np.searchsorted(uk, k)] += 1
-    freq = counts / nsamp
-    # successive samples are correlated; allow 6 sigma with an inflation of 4
-    sig = np.sqrt(exact * (1 - exact) / nsamp) * 4 + 1e-4
-    assert np.all(np.abs(freq - exact) < 6 * sig), np.max(np.abs(freq - exact) / sig)
-
-
-# ------------------------------------------------------- collective variables
-def _periodic_largest(grid, spin):
-    from scipy import ndimage
-
-    m = (grid == spin)
-    lab, n = ndimage.label(m)
-    if n == 0:
-        return 0
-    parent = list(range(n + 1))
-
-    def find(a):
-        while parent[a] != a:
-            parent[a] = parent[parent[a]]
-            a = parent[a]
-        return a
-
-    Lr, Lc = grid.shape
-    for r in range(Lr):
-        a, b = lab[r, 0], lab[r, Lc - 1]
-        if a and b:
-            parent[find(a)] = find(b)
-    for c in range(Lc):
-        a, b = lab[0, c], lab[Lr - 1, c]
-        if a and b:
-            parent[find(a)] = find(b)
-    sizes = {}
-    for v in lab[lab > 0]:
-        rt = find(int(v))
-        sizes[rt] = sizes.get(rt, 0) + 1
-    return max(sizes.values())
+    L, beta, h, R, seed = 3, 0.35, 0.2, 32, 77
+    exact = _boltzmann_3x3(beta, h)
+    g = -np.ones((R, L, L), dtype=np.int32)
+    oracle.sweep_philox(L, g, seed, 0, 200 * 9, beta, h, nthreads=1)  # burn-in
+    nb, per = 40, 500
+    freq = np.zeros((nb, 512))
+    m_series = []
+    a = 200 * 9
+    for b in range(nb):
+        for _ in range(per):
+            oracle.sweep_philox(L, g, seed, a, 9, beta, h, nthreads=1)
+            a += 9
+            freq[b] += np.bincount(_state_index(g), minlength=512)
+            m_series.append(g[0].sum())
+        freq[b] /= per * R
+    mean = freq.mean(0)
+    se = freq.std(0, ddof=1) / np.sqrt(nb)
+    big = exact * per * R >= 20
+    z = (mean[big] - exact[big]) / se[big]
+    assert big.sum() > 50
+    assert np.max(np.abs(z)) < 5.5, np.max(np.abs(z))
+    assert 0.6 < np.mean(z * z) < 1.6, np.mean(z * z)
+    rare = freq[:, ~big].sum(1)
+    assert abs(rare.mean() - exact[~big].sum()) < 5 * rare.std(ddof=1) / np.sqrt(nb)
+    x = np.array(m_series, dtype=float)
+    tau = per * x.reshape(nb, per).mean(1).var(ddof=1) / x.var() / 2
+    print(f"tau_int(m) = {tau:.2f} sweeps")
+    assert tau < per / 10
 
 
 def test_largest_cluster_matches_reference_and_scipy(oracle, gold):
