@@ -464,7 +464,8 @@ def main() -> int:
     # methods 1/2: LDS bytes (5 reads of the 4-byte bank word holding the spin bit); method 0: HBM
     # bytes (5 spin bytes read per attempt)
     hbm = args.method == 0
-    bpa = HBM_BYTES_PER_ATTEMPT if hbm else LDS_BYTES_PER_ATTEMPT
+    # method 3 (bit-sliced): five 8-byte word reads per attempt serve the word's 64 replicas
+    bpa = HBM_BYTES_PER_ATTEMPT if hbm else (5 * 8 / 64 if args.method == 3 else LDS_BYTES_PER_ATTEMPT)
     peak = HBM_PEAK_GBS if hbm else LDS_PEAK_GBS
     achieved = SWEEPS_PER_STEP * L * L * R * bpa / (per_launch_ms / 1e3) / 1e9
     multi = {1: "LAT_MULTISPIN", 2: "LAT_MULTIHALF", 4: "LAT_MULTI16", 8: "LAT_MULTI8"}
@@ -504,7 +505,8 @@ def main() -> int:
             "frac": achieved / peak,
             "traffic": None,
             "bytes_per_attempt": bpa,
-            "kernel": f"sweep_kernel<{kname}, RNG_PHILOX, S>",
+            "kernel": (f"sweep_sliced<{64 // max(split, 1)}, {S}, true>" if args.method == 3
+                       else f"sweep_kernel<{kname}, RNG_PHILOX, S>"),
             "kernel_ms_per_launch": per_launch_ms,
             "kernel_attempts_per_s": kern_rate,
         },

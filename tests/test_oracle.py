@@ -244,6 +244,38 @@ def test_boltzmann_3x3_exact_enumeration(oracle):
     assert tau < per / 10
 
 
+def _periodic_largest(grid, spin):
+    from scipy import ndimage
+
+    m = (grid == spin)
+    lab, n = ndimage.label(m)
+    if n == 0:
+        return 0
+    parent = list(range(n + 1))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    Lr, Lc = grid.shape
+    for r in range(Lr):
+        a, b = lab[r, 0], lab[r, Lc - 1]
+        if a and b:
+            parent[find(a)] = find(b)
+    for c in range(Lc):
+        a, b = lab[0, c], lab[Lr - 1, c]
+        if a and b:
+            parent[find(a)] = find(b)
+    sizes = {}
+    for v in lab[lab > 0]:
+        rt = find(int(v))
+        sizes[rt] = sizes.get(rt, 0) + 1
+    return max(sizes.values())
+
+
+
 def test_largest_cluster_matches_reference_and_scipy(oracle, gold):
     off = 0
     for i, L in enumerate(gold["clus_Ls"]):
