@@ -75,5 +75,7 @@ def test_metastable_mode_and_nucleation_rate(gpu, engine_lib):
     f = np.array(frac)
     k, t0 = curve_fit(lambda t, k, t0: 1 - np.exp(-k * (t - t0)), t[f > 0], f[f > 0], p0=(1.5e-4, 500))[0]
     print(f"k = {k:.4e} per sweep, t0 = {t0:.0f}", flush=True)
-    # the reference's k comes from 1024 replicas (about 1000 events): ~4% standard error
-    assert abs(k - 1.5032e-4) < 0.15 * 1.5032e-4, k
+    # The reference's k comes from 1024 replicas (about 1000 events): ~3.2%
+    # standard error; this fit over 16384 replicas and 20000 sweeps (about
+    # 15000 events) adds ~1%.  2.5 combined standard errors: 8%.
+    assert abs(k - 1.5032e-4) < 0.08 * 1.5032e-4, k
