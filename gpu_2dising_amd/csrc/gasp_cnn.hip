@@ -19,7 +19,7 @@
 //   forward / backward-data:  D[cout][pos] += W^T[cout][cin] X[cin][pos], one
 //     MFMA per tap and 16-cout half: 18 per 16-position tile.
 //   weight gradient:          D[cout][cin] += dZ^T[cout][pos] X_t[pos][cin] over
-//     32-position chunks staged transposed in LDS.
+//     32-position chunks staged transposed in LDS, all nine taps per chunk.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -189,135 +189,307 @@ __global__ void __launch_bounds__(256) conv_mfma(const uint16_t *__restrict__ in
 }
 
 // ------------------------------------------------------------------ weight gradient
-// grid (n, 9 taps), 4 waves per block, each wave a quarter of the image's
-// positions.  Writes the wave's partial dW[t][ci][co] (and, for the centre
-// tap, the partial bias gradient) to part[(img * 4 + wave)][...].
+// dW[t][ci][co] = sum over positions p of a[p + off_t][ci] dz[p][co]: a GEMM
+// whose K is the positions.  A workgroup takes bands of 4 rows x XW = min(L,
+// 64) columns of one image in turn.  Per band it stages, through registers,
+// dz transposed (zT[co][row][x]) and the input rows y0 - 1 .. y0 + 4
+// transposed with one periodic halo column each side (aT[ci][row][8 + x],
+// x = -1 .. XW): a thread loads four positions x eight channels (four 16-byte
+// loads) and writes eight 8-byte channel runs.  The band's positions then go
+// to the waves as K-chunks of 32 (four runs of 8): the A fragment (dz^T, 8
+// positions of one co per lane) is one 16-byte LDS read; for each kernel row
+// one 16-byte read plus the dword either side give the B fragments of the
+// taps dx = -1, 0, +1 (positions shifted by one bf16 with v_alignbit).  Waves
+// 2 j and 2 j + 1 take the same K-chunks for ci half 0 and 1, so a K-chunk
+// costs a wave 2 + 3 x 3 LDS reads for 18 MFMAs (nine taps, two co halves)
+// and the nine taps' accumulators (72 registers) leave room for three
+// workgroups per CU (160 registers a lane, 49 KB of LDS each).  They stay in registers over all of the workgroup's
+// bands; at the end the four waves add theirs in a fixed order through LDS
+// and the workgroup writes one partial row part[blk] = [dW[t][ci][co] |
+// db[co] (sum of dz)], the order of the parameter vector, so one reduction
+// covers both and the result does not depend on scheduling.  Channel runs are 260 (aT) and 132 (zT) dwords apart,
+// 4 mod 64: the 16 lanes of a fragment read start on different bank quads.
+constexpr int kWgXW = 64;                   // band width (columns)
+constexpr int kWgARow = 40, kWgAPitch = 260;  // aT: dwords per row (8 + 64 + 8 bf16), per channel (6 rows + 20)
+constexpr int kWgZRow = 32, kWgZPitch = 132;  // zT: dwords per row (64 bf16), per channel (4 rows + 4)
+constexpr int kWgBlocks = 768;              // workgroups (three per CU on 256 CUs)
+
+__device__ __forceinline__ uint32_t bf_pair(uint32_t hi_src, uint32_t lo_src, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi_src, lo_src, sel);
+}
+
+// channel c = cb + 2 q + s of four positions' dwords q: s = 0 the low halves, 1 the high
+__device__ constexpr uint32_t kSel[2] = {0x05040100u, 0x07060302u};
+
+// zT[co][rr][x] = dz of band row rr (image row y0 + rr), column x0 + x, for
+// rr < 4 and x < XW: items of 4 positions x 8 channels (four 16-byte loads,
+// eight 8-byte channel runs), over the workgroup's 256 threads.
+__device__ __forceinline__ void stage_dz_t(uint32_t *zT, const uint16_t *__restrict__ zb, int L, int y0, int x0,
+                                           int XW, int tid) {
+    for (int it = tid; it < 4 * XW; it += 256) {
+        const int cb = (it & 3) * 8, pq = it >> 2;
+        const int rr = pq / (XW / 4), x = 4 * (pq % (XW / 4));
+        const uint4 *src = (const uint4 *)(zb + ((size_t)(y0 + rr) * L + x0 + x) * C + cb);
+        const uint4 v[4] = {src[0], src[4], src[8], src[12]};  // positions x .. x + 3
+        const uint32_t d[4][4] = {{v[0].x, v[0].y, v[0].z, v[0].w}, {v[1].x, v[1].y, v[1].z, v[1].w},
+                                  {v[2].x, v[2].y, v[2].z, v[2].w}, {v[3].x, v[3].y, v[3].z, v[3].w}};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int sh = 0; sh < 2; ++sh) {
+                uint2 run;
+                run.x = bf_pair(d[1][q], d[0][q], kSel[sh]);
+                run.y = bf_pair(d[3][q], d[2][q], kSel[sh]);
+                *(uint2 *)&zT[(cb + 2 * q + sh) * kWgZPitch + rr * kWgZRow + x / 2] = run;
+            }
+    }
+}
+
+constexpr int kWgRow = 9 * C * C + C;  // partial row: weights then bias
+static_assert(OFF_B[2] == OFF_W[2] + 9 * C * C && OFF_B[3] == OFF_W[3] + 9 * C * C && OFF_B[4] == OFF_W[4] + 9 * C * C,
+              "each conv's bias follows its weights");
+
 __global__ void __launch_bounds__(256) conv_wgrad(const uint16_t *__restrict__ ain, const uint16_t *__restrict__ dz,
-                                                  float *__restrict__ part, float *__restrict__ part_b, int n, int L) {
-    __shared__ uint16_t sz[4][C][40];  // [wave][channel][32 positions + pad]
-    __shared__ uint16_t sa[4][C][40];
-    const int img = blockIdx.x, t = blockIdx.y;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int N = L * L, m = L - 1, dy = t / 3 - 1, dx = t % 3 - 1;
-    const int chunk = N / 4;  // positions per wave (multiple of 32 for L >= 16)
-    const uint16_t *zb = dz + (size_t)img * N * C;
-    const uint16_t *ab = ain + (size_t)img * N * C;
-    f32x4 acc[2][2] = {{{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}}, {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}}};
-    float bsum[2] = {0.f, 0.f};
+                                                  float *__restrict__ part, int n, int L) {
+    __shared__ __align__(16) uint32_t lds[C * kWgAPitch + C * kWgZPitch];
+    uint32_t *aT = lds, *zT = lds + C * kWgAPitch;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int N = L * L, m = L - 1;
+    const int XW = L < kWgXW ? L : kWgXW, nxc = L / XW;
+    const int nbands = n * (L / 4) * nxc;
     const int pl = lane & 15, kq = lane >> 4;
-    for (int p0 = w * chunk; p0 < (w + 1) * chunk; p0 += 32) {
-        // stage 32 positions x 32 channels of dz and of the shifted input, transposed
+    const int nh = w & 1;  // ci half of this wave
+    f32x4 acc[9][2];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int item = lane + 64 * r;  // 128 items: position (item >> 2), channel block (item & 3)
-            const int k = item >> 2, cb = (item & 3) * 8;
-            const int pos = p0 + k;
-            const int yy = ((pos / L) + dy) & m, xx = ((pos % L) + dx) & m;
-            const uint4 vz = *(const uint4 *)(zb + (size_t)pos * C + cb);
-            const uint4 va = *(const uint4 *)(ab + (size_t)(yy * L + xx) * C + cb);
-            const uint32_t z4[4] = {vz.x, vz.y, vz.z, vz.w}, a4[4] = {va.x, va.y, va.z, va.w};
+    for (int t = 0; t < 9; ++t)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                sz[w][cb + 2 * q][k] = (uint16_t)z4[q];
-                sz[w][cb + 2 * q + 1][k] = (uint16_t)(z4[q] >> 16);
-                sa[w][cb + 2 * q][k] = (uint16_t)a4[q];
-                sa[w][cb + 2 * q + 1][k] = (uint16_t)(a4[q] >> 16);
+        for (int h = 0; h < 2; ++h) acc[t][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[2] = {0.f, 0.f};
+    for (int band = blockIdx.x; band < nbands; band += gridDim.x) {
+        const int img = band / ((L / 4) * nxc), rem = band % ((L / 4) * nxc);
+        const int y0 = 4 * (rem / nxc), x0 = XW * (rem % nxc);
+        const uint16_t *zb = dz + (size_t)img * N * C;
+        const uint16_t *ab = ain + (size_t)img * N * C;
+        __syncthreads();  // the previous band's fragments have been read
+        stage_dz_t(zT, zb, L, y0, x0, XW, tid);
+        // input: rows y0 - 1 .. y0 + 4 x positions x0 .. x0 + XW - 1, then the halo columns
+        for (int it = tid; it < 6 * XW + 48; it += 256) {
+            if (it < 6 * XW) {
+                const int cb = (it & 3) * 8, pq = it >> 2;
+                const int r = pq / (XW / 4), x = 4 * (pq % (XW / 4));
+                const int yy = (y0 + r - 1) & m;
+                const uint4 *src = (const uint4 *)(ab + ((size_t)yy * L + x0 + x) * C + cb);
+                const uint4 v[4] = {src[0], src[4], src[8], src[12]};
+                const uint32_t d[4][4] = {{v[0].x, v[0].y, v[0].z, v[0].w}, {v[1].x, v[1].y, v[1].z, v[1].w},
+                                          {v[2].x, v[2].y, v[2].z, v[2].w}, {v[3].x, v[3].y, v[3].z, v[3].w}};
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int sh = 0; sh < 2; ++sh) {
+                        uint2 run;
+                        run.x = bf_pair(d[1][q], d[0][q], kSel[sh]);
+                        run.y = bf_pair(d[3][q], d[2][q], kSel[sh]);
+                        *(uint2 *)&aT[(cb + 2 * q + sh) * kWgAPitch + r * kWgARow + (8 + x) / 2] = run;
+                    }
+            } else {
+                const int hq = it - 6 * XW;  // 48 items: (row, side, channel block)
+                const int cb = (hq & 3) * 8, side = (hq >> 2) & 1, r = hq >> 3;
+                const int yy = (y0 + r - 1) & m;
+                const int xx = side ? ((x0 + XW) & m) : ((x0 - 1) & m);
+                const int idx = side ? 8 + XW : 7;  // bf16 index in the row
+                const uint4 v = *(const uint4 *)(ab + ((size_t)yy * L + xx) * C + cb);
+                const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+                uint16_t *a16 = (uint16_t *)aT;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    a16[(size_t)(cb + 2 * q) * 2 * kWgAPitch + r * 2 * kWgARow + idx] = (uint16_t)d[q];
+                    a16[(size_t)(cb + 2 * q + 1) * 2 * kWgAPitch + r * 2 * kWgARow + idx] = (uint16_t)(d[q] >> 16);
+                }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint4 A[2], B[2];
+        __syncthreads();
+        // K-chunks of 32 positions: runs g = 4 kc + kq of 8 positions (row g / (XW / 8))
+        for (int kc = w >> 1; kc < XW / 8; kc += 2) {
+            const int g = 4 * kc + kq;
+            const int rr = g / (XW / 8), xg = 8 * (g % (XW / 8));
+            uint4 A[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            A[h] = *(const uint4 *)&sz[w][16 * h + pl][8 * kq];
-            B[h] = *(const uint4 *)&sa[w][16 * h + pl][8 * kq];
+            for (int h = 0; h < 2; ++h) A[h] = *(const uint4 *)&zT[(16 * h + pl) * kWgZPitch + rr * kWgZRow + xg / 2];
+            if (nh == 0) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t v4[4] = {A[h].x, A[h].y, A[h].z, A[h].w};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) bsum[h] += bf2f((uint16_t)v4[q]) + bf2f((uint16_t)(v4[q] >> 16));
+                }
+            }
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int base = (16 * nh + pl) * kWgAPitch + (rr + dy) * kWgARow + (8 + xg) / 2;
+                const uint4 Q = *(const uint4 *)&aT[base];
+                const uint32_t Dp = aT[base - 1], Dn = aT[base + 4];
+                uint4 Bm, Bp;
+                Bm.x = __builtin_amdgcn_alignbit(Q.x, Dp, 16);
+                Bm.y = __builtin_amdgcn_alignbit(Q.y, Q.x, 16);
+                Bm.z = __builtin_amdgcn_alignbit(Q.z, Q.y, 16);
+                Bm.w = __builtin_amdgcn_alignbit(Q.w, Q.z, 16);
+                Bp.x = Bm.y;
+                Bp.y = Bm.z;
+                Bp.z = Bm.w;
+                Bp.w = __builtin_amdgcn_alignbit(Dn, Q.w, 16);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    acc[3 * dy][h] = mfma(A[h], Bm, acc[3 * dy][h]);
+                    acc[3 * dy + 1][h] = mfma(A[h], Q, acc[3 * dy + 1][h]);
+                    acc[3 * dy + 2][h] = mfma(A[h], Bp, acc[3 * dy + 2][h]);
+                }
+            }
         }
-        if (t == 4) {
+    }
+
+    // the waves' sums in a fixed order through LDS: waves 0 and 1 store
+    // (ci halves 0 and 1), then waves 2 and 3 add
+    float *red = (float *)lds;  // [9][ci][co] then the 32 bias sums
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float v = bsum[h];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        bsum[h] = v;  // lanes pl hold channel 16 h + pl
+    }
+    __syncthreads();
+    for (int k = 0; k < 2; ++k) {
+        if ((w >> 1) == k) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int co = 16 * h + 4 * kq + j, ci = 16 * nh + pl;
+                        float *d = &red[t * C * C + ci * C + co];
+                        *d = (k == 0) ? acc[t][h][j] : *d + acc[t][h][j];
+                    }
+            // bias sums: waves 0 (stores) and 2 (adds)
+            if (nh == 0 && kq == 0) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    float *d = &red[9 * C * C + 16 * h + pl];
+                    *d = (k == 0) ? bsum[h] : *d + bsum[h];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float *out = part + (size_t)blockIdx.x * kWgRow;
+    for (int i = tid; i < kWgRow; i += 256) out[i] = red[i];
+}
+
+// ------------------------------------------------------------------ conv1 weight gradient
+// dW1[t][c] = sum over positions p of s[p + off_t] dz1[p][c] and db1[c] =
+// sum of dz1[p][c], on MFMA as D[c][t] += dz1^T[c][p] S[p][t] with the nine
+// taps as N (lanes t >= 9 hold zeros).  Workgroups take bands of 4 rows x XW
+// columns as conv_wgrad does: dz1 staged transposed (stage_dz_t), the spins
+// of rows y0 - 1 .. y0 + 4 as bfloat16 +-1 with a periodic halo column each
+// side.  Lane t's B fragment is 8 spins of its tap's row, shifted by its dx
+// with v_alignbit (a per-lane shift).  The waves take the K-chunks in turn,
+// add their sums in a fixed order through LDS, and each workgroup writes one
+// partial [w1 (9 x 32) | b1 (32)] to part1[blk][320].
+constexpr int kW1Blocks = 1024;
+
+__global__ void __launch_bounds__(256) conv1_wgrad(const int8_t *__restrict__ s, const uint16_t *__restrict__ dz,
+                                                   float *__restrict__ part1, int n, int L) {
+    __shared__ __align__(16) uint32_t zT[C * kWgZPitch];
+    __shared__ __align__(16) uint32_t sT[6 * kWgARow];  // [row][8 + x] bfloat16
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int N = L * L, m = L - 1;
+    const int XW = L < kWgXW ? L : kWgXW, nxc = L / XW;
+    const int nbands = n * (L / 4) * nxc;
+    const int pl = lane & 15, kq = lane >> 4;
+    // this lane's tap (t < 9; lanes 9..15 of each group contribute zeros)
+    const bool tap_ok = pl < 9;
+    const int tt = tap_ok ? pl : 4;
+    const int tdy = tt / 3, tdx = tt % 3 - 1;
+    f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    float bsum[2] = {0.f, 0.f};
+
+    for (int band = blockIdx.x; band < nbands; band += gridDim.x) {
+        const int img = band / ((L / 4) * nxc), rem = band % ((L / 4) * nxc);
+        const int y0 = 4 * (rem / nxc), x0 = XW * (rem % nxc);
+        __syncthreads();  // the previous band's fragments have been read
+        stage_dz_t(zT, dz + (size_t)img * N * C, L, y0, x0, XW, tid);
+        uint16_t *s16 = (uint16_t *)sT;
+        for (int it = tid; it < 6 * (XW + 2); it += 256) {
+            const int r = it / (XW + 2), xi = it % (XW + 2);  // xi = x + 1, x = -1 .. XW
+            const int yy = (y0 + r - 1) & m, xx = (x0 + xi - 1) & m;
+            const int8_t v = s[(size_t)img * N + (size_t)yy * L + xx];
+            s16[r * 2 * kWgARow + 7 + xi] = v > 0 ? (uint16_t)0x3F80u : (uint16_t)0xBF80u;
+        }
+        __syncthreads();
+        for (int kc = w; kc < XW / 8; kc += 4) {
+            const int g = 4 * kc + kq;
+            const int rr = g / (XW / 8), xg = 8 * (g % (XW / 8));
+            uint4 A[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) A[h] = *(const uint4 *)&zT[(16 * h + pl) * kWgZPitch + rr * kWgZRow + xg / 2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const uint32_t v4[4] = {A[h].x, A[h].y, A[h].z, A[h].w};
 #pragma unroll
                 for (int q = 0; q < 4; ++q) bsum[h] += bf2f((uint16_t)v4[q]) + bf2f((uint16_t)(v4[q] >> 16));
             }
-        }
+            const int base = (rr + tdy) * kWgARow + (8 + xg) / 2;
+            const uint4 Q = *(const uint4 *)&sT[base];
+            const uint32_t Dp = sT[base - 1], Dn = sT[base + 4];
+            // dx = -1: (Q_i : Q_i-1) >> 16; 0: Q_i; +1: (Q_i+1 : Q_i) >> 16
+            const uint32_t q[6] = {Dp, Q.x, Q.y, Q.z, Q.w, Dn};
+            const uint32_t sh = tdx == 0 ? 0u : 16u;
+            uint32_t b[4];
 #pragma unroll
-        for (int mh = 0; mh < 2; ++mh)
-#pragma unroll
-            for (int nh = 0; nh < 2; ++nh) acc[mh][nh] = mfma(A[mh], B[nh], acc[mh][nh]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    float *out = part + ((size_t)(img * 4 + w) * 9 + t) * C * C;
-#pragma unroll
-    for (int mh = 0; mh < 2; ++mh)
-#pragma unroll
-        for (int nh = 0; nh < 2; ++nh)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int co = 16 * mh + 4 * kq + j, ci = 16 * nh + pl;
-                out[ci * C + co] = acc[mh][nh][j];
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t lo = tdx < 0 ? q[i] : q[i + 1];
+                const uint32_t hi = tdx > 0 ? q[i + 2] : q[i + 1];
+                b[i] = tap_ok ? __builtin_amdgcn_alignbit(hi, lo, sh) : 0u;
             }
-    if (t == 4) {
-        // lanes l, l+16, l+32, l+48 hold partial sums of channel 16 h + l % 16
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float v = bsum[h];
-            v += __shfl_xor(v, 16, 64);
-            v += __shfl_xor(v, 32, 64);
-            if (kq == 0) part_b[(size_t)(img * 4 + w) * C + 16 * h + pl] = v;
+            const uint4 B = make_uint4(b[0], b[1], b[2], b[3]);
+            acc[0] = mfma(A[0], B, acc[0]);
+            acc[1] = mfma(A[1], B, acc[1]);
         }
     }
-}
 
-// ------------------------------------------------------------------ conv1 weight gradient
-// One block per (image, row), wave = channel octet, lane = column.  Partials
-// [w1 (9 x 32) | b1 (32)] per row to part1[(img * L + y)][320].
-__global__ void __launch_bounds__(256) conv1_wgrad(const int8_t *__restrict__ s, const uint16_t *__restrict__ dz,
-                                                   float *__restrict__ part1, int n, int L) {
-    const int row = blockIdx.x;
-    const int img = row / L, y = row % L, m = L - 1, N = L * L;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float acc[9][8], bacc[8];
+    // the waves' sums in a fixed order (wave 0, 1, 2, 3) through LDS
+    float *red = (float *)zT;  // [9][32] then the 32 bias sums
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        bacc[c] = 0.f;
-#pragma unroll
-        for (int t = 0; t < 9; ++t) acc[t][c] = 0.f;
+    for (int h = 0; h < 2; ++h) {
+        float v = bsum[h];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        bsum[h] = v;
     }
-    for (int x = lane; x < L; x += 64) {
-        float sv[9];
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+        if (w == k) {
+            if (tap_ok) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
-            sv[t] = (float)s[(size_t)img * N + ((y + t / 3 - 1) & m) * L + ((x + t % 3 - 1) & m)];
-        const uint4 v = *(const uint4 *)(dz + ((size_t)img * N + y * L + x) * C + 8 * w);
-        const uint32_t d4[4] = {v.x, v.y, v.z, v.w};
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float d0 = bf2f((uint16_t)d4[q]), d1 = bf2f((uint16_t)(d4[q] >> 16));
-            bacc[2 * q] += d0;
-            bacc[2 * q + 1] += d1;
+                    for (int j = 0; j < 4; ++j) {
+                        float *d = &red[pl * C + 16 * h + 4 * kq + j];
+                        *d = (k == 0) ? acc[h][j] : *d + acc[h][j];
+                    }
+            }
+            if (kq == 0) {
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                acc[t][2 * q] += sv[t] * d0;
-                acc[t][2 * q + 1] += sv[t] * d1;
+                for (int h = 0; h < 2; ++h) {
+                    float *d = &red[288 + 16 * h + pl];
+                    *d = (k == 0) ? bsum[h] : *d + bsum[h];
+                }
             }
         }
+        __syncthreads();
     }
-    float *out = part1 + (size_t)row * 320;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            float v = acc[t][c];
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) out[t * C + 8 * w + c] = v;
-        }
-        float b = bacc[c];
-        for (int off = 32; off >= 1; off >>= 1) b += __shfl_xor(b, off, 64);
-        if (lane == 0) out[288 + 8 * w + c] = b;
-    }
+    float *out = part1 + (size_t)blockIdx.x * 320;
+    for (int i = tid; i < 320; i += 256) out[i] = red[i];
 }
 
 // ------------------------------------------------------------------ head
@@ -328,19 +500,35 @@ __global__ void __launch_bounds__(256) head(const uint16_t *__restrict__ a4, con
                                             const float *__restrict__ labels, float *__restrict__ pb,
                                             float *__restrict__ loss, float *__restrict__ part_head,
                                             float *__restrict__ dg, int n, int L, float inv_batch) {
-    __shared__ float red[8][C];
+    __shared__ float red[4][C];
     __shared__ float g[C], hpre[HID], h[HID], dh[HID], zsh;
     const int img = blockIdx.x, tid = threadIdx.x, N = L * L;
-    const int c = tid & 31, grp = tid >> 5;
-    float sum = 0.f;
-    for (int q = grp; q < N; q += 8) sum += bf2f(a4[((size_t)img * N + q) * C + c]);
-    red[grp][c] = sum;
-    __syncthreads();
-    if (tid < C) {
-        float s = 0.f;
-        for (int k = 0; k < 8; ++k) s += red[k][tid];
-        g[tid] = s / (float)N;
+    // pool: thread = (position lane tid / 4, channel octet tid % 4), one
+    // 16-byte load of 8 channels per position
+    const int oc = tid & 3, pg = tid >> 2;
+    float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const uint4 *src = (const uint4 *)(a4 + (size_t)img * N * C) + oc;
+#pragma unroll 4
+    for (int q = pg; q < N; q += 64) {
+        const uint4 v = src[(size_t)q * (C / 8)];
+        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            sum[2 * k] += bf2f((uint16_t)d[k]);
+            sum[2 * k + 1] += bf2f((uint16_t)(d[k] >> 16));
+        }
     }
+    // lanes with the same octet: xor over the position bits of the lane
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int off = 4; off < 64; off <<= 1) sum[k] += __shfl_xor(sum[k], off, 64);
+    if ((tid & 63) < 4) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) red[tid >> 6][8 * oc + k] = sum[k];
+    }
+    __syncthreads();
+    if (tid < C) g[tid] = (red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]) / (float)N;
     __syncthreads();
     if (tid < HID) {
         float v = p[OFF_FB1 + tid];
@@ -455,7 +643,7 @@ struct gasp_cnn {
     float *y = nullptr, *pb = nullptr, *loss = nullptr, *dg = nullptr;
     uint16_t *a[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // a[1..4]
     uint16_t *dz0 = nullptr, *dz1 = nullptr;
-    float *part = nullptr, *part_b = nullptr, *part1 = nullptr, *part_head = nullptr;
+    float *part = nullptr, *part1 = nullptr, *part_head = nullptr;
     float *red = nullptr;  // chunk sums of the two-level reductions
     int8_t *data = nullptr;
     float *data_y = nullptr;
@@ -484,7 +672,7 @@ extern "C" int gasp_cnn_destroy(gasp_cnn *c) {
     for (void *q : {(void *)c->p, (void *)c->g, (void *)c->m, (void *)c->v, (void *)c->w1r, (void *)c->wf,
                     (void *)c->wb, (void *)c->s, (void *)c->y, (void *)c->pb, (void *)c->loss, (void *)c->dg,
                     (void *)c->a[1], (void *)c->a[2], (void *)c->a[3], (void *)c->a[4], (void *)c->dz0,
-                    (void *)c->dz1, (void *)c->part, (void *)c->part_b, (void *)c->part1, (void *)c->part_head,
+                    (void *)c->dz1, (void *)c->part, (void *)c->part1, (void *)c->part_head,
                     (void *)c->red, (void *)c->data, (void *)c->data_y, (void *)c->order})
         cfree(q);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -523,13 +711,12 @@ extern "C" int gasp_cnn_create(int L, int max_batch, gasp_cnn **out) {
     for (int k = 1; k <= 4; ++k) A((void **)&c->a[k], n * N * C * 2);
     A((void **)&c->dz0, n * N * C * 2);
     A((void **)&c->dz1, n * N * C * 2);
-    A((void **)&c->part, n * 4 * 9 * C * C * 4);
-    A((void **)&c->part_b, n * 4 * C * 4);
+    A((void **)&c->part, n * 4 * kWgRow * 4);
     A((void **)&c->part1, n * (size_t)L * 320 * 4);
     A((void **)&c->part_head, n * HEAD_N * 4);
     {
         const size_t q = (size_t)kRedChunk;
-        const size_t r = std::max({(n * 4 + q - 1) / q * 9 * C * C, (n * L + q - 1) / q * 320, (n + q - 1) / q * HEAD_N});
+        const size_t r = std::max({(n * 4 + q - 1) / q * kWgRow, (n * L + q - 1) / q * 320, (n + q - 1) / q * HEAD_N});
         A((void **)&c->red, r * 4);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -615,16 +802,23 @@ static int backward(gasp_cnn *c, int n) {
                        c->dz0, n, L);
     uint16_t *dz = c->dz0, *dzn = c->dz1;
     for (int k = 4; k >= 2; --k) {
-        hipLaunchKernelGGL(conv_wgrad, dim3(n, 9), dim3(256), 0, c->st, c->a[k - 1], dz, c->part, c->part_b, n, L);
-        if ((rc = reduce_to(c, c->part, n * 4, 9 * C * C, c->g + OFF_W[k])) != 0) return rc;
-        if ((rc = reduce_to(c, c->part_b, n * 4, C, c->g + OFF_B[k])) != 0) return rc;
+        // one partial per workgroup; the buffers hold 4 nmax of them
+        const int nbands = n * (L / 4) * (L / std::min(L, kWgXW));
+        const int nblk = std::min({nbands, kWgBlocks, 4 * c->nmax});
+        hipLaunchKernelGGL(conv_wgrad, dim3(nblk), dim3(256), 0, c->st, c->a[k - 1], dz, c->part, n, L);
+        // weights and bias are adjacent in the parameter vector
+        if ((rc = reduce_to(c, c->part, nblk, kWgRow, c->g + OFF_W[k])) != 0) return rc;
         // d a_{k-1}, masked by a_{k-1} > 0
         hipLaunchKernelGGL(conv_mfma<true>, dim3(rows / kConvRows), dim3(256), 0, c->st, dz,
                            c->wb + (size_t)(k - 2) * 9 * 2 * 64 * 8, (const float *)nullptr, c->a[k - 1], dzn, n, L);
         std::swap(dz, dzn);
     }
-    hipLaunchKernelGGL(conv1_wgrad, dim3(rows), dim3(256), 0, c->st, c->s, dz, c->part1, n, L);
-    if ((rc = reduce_to(c, c->part1, rows, 320, c->g)) != 0) return rc;
+    {
+        const int nbands = n * (L / 4) * (L / std::min(L, kWgXW));
+        const int nblk = std::min({nbands, kW1Blocks, rows});  // part1 holds n L partials
+        hipLaunchKernelGGL(conv1_wgrad, dim3(nblk), dim3(256), 0, c->st, c->s, dz, c->part1, n, L);
+        if ((rc = reduce_to(c, c->part1, nblk, 320, c->g)) != 0) return rc;
+    }
     CTRY(hipGetLastError());
     return GASP_OK;
 }

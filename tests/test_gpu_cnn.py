@@ -33,7 +33,7 @@ def test_forward_matches_oracle(gpu, engine_lib, L, n):
     assert np.max(np.abs(got - ref)) < 2e-3, np.max(np.abs(got - ref))
 
 
-@pytest.mark.parametrize("L,n", [(16, 6), (32, 4)])
+@pytest.mark.parametrize("L,n", [(16, 6), (32, 4), (64, 3), (128, 2)])
 def test_gradient_matches_oracle(gpu, engine_lib, L, n):
     K, p, spins, y = _case(L, n, 2)
     print(f"[cnn grad] L={L} n={n}", flush=True)
