@@ -47,6 +47,9 @@ constexpr int kLdsThr = 0;
 constexpr int kLdsCnt = 128;
 constexpr int kLdsNbr = 384;
 constexpr int kLdsLattice = 912;  // 384 + 65 masks of 8 bytes, rounded up to 16
+// Fourth dword of a raw buffer resource on gfx9 parts (no format conversion;
+// byte offsets checked against the record count).
+constexpr int kBufferWord3 = 0x00020000;
 
 // LDS by absolute address.  The sweep kernels declare no static LDS, so their
 // dynamic LDS (smem) starts at LDS address 0 and smem + x is address x.  A
@@ -229,13 +232,18 @@ struct Geometry {
 // Layout 1 (bit lattice): replica-private region of `stride` dwords in LDS,
 //   bit (site & 31) of dword (site >> 5).
 // Layout 0 (HBM): one byte (0/1) per site in global memory, replica stride
-//   `stride` bytes; accessed with workgroup-scope atomics so that the lanes of
-//   a replica see each other's flips.
+//   `stride` bytes.  A workgroup's replicas are one buffer resource (scalar
+//   registers) and a lane addresses its replica's bytes by a 32-bit offset:
+//   no 64-bit address arithmetic per read, and reads past the workgroup's
+//   replicas return 0.  The sweeps order a round's flips before the next
+//   round's reads with fences (one wave) or barriers (several).
 template <int LAT>
 struct Lattice {
     unsigned char *base;  // LDS (1/2) or global (0) base of this lane's replica
     unsigned *dw;         // layout 2: LDS lattice as dwords
     int rl;               // layout 2: replica bit within the 64-bit word
+    __amdgpu_buffer_rsrc_t rsrc;  // layout 0: the workgroup's replicas
+    unsigned off;                 // layout 0: byte offset of this lane's replica in rsrc
 
     __device__ __forceinline__ int spin(int site) const {
         if constexpr (LAT == LAT_MULTISPIN) {
@@ -251,7 +259,7 @@ struct Lattice {
         } else if constexpr (LAT == LAT_BITS) {
             return (base[site >> 3] >> (site & 7)) & 1;
         } else {
-            return __hip_atomic_load(base + site, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return (int)__builtin_amdgcn_raw_buffer_load_b8(rsrc, off + (unsigned)site, 0, 0);
         }
     }
     __device__ __forceinline__ void flip(int site, int old) const {
@@ -265,8 +273,7 @@ struct Lattice {
         } else if constexpr (LAT == LAT_BITS) {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
         } else {
-            __hip_atomic_store(base + site, (unsigned char)(old ^ 1), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(old ^ 1), rsrc, off + (unsigned)site, 0, 0);
         }
     }
 };
@@ -296,11 +303,9 @@ __device__ __forceinline__ Spins read_spins(const Lattice<LAT> &lat, const Stenc
         const bool wl = st.lt != c - 1, wr = st.rt != c + 1;  // column 0 / L - 1
         const int lo = wl ? c : c - 1, hi = wr ? c : c + 1;
         const int d0 = lo & ~3;
-        const unsigned *dw = (const unsigned *)lat.base;
-        const unsigned v0 = __hip_atomic_load(dw + (d0 >> 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const unsigned v0 = __builtin_amdgcn_raw_buffer_load_b32(lat.rsrc, lat.off + (unsigned)d0, 0, 0);
         unsigned v1 = 0u;
-        if ((hi & ~3) != d0)
-            v1 = __hip_atomic_load(dw + (d0 >> 2) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((hi & ~3) != d0) v1 = __builtin_amdgcn_raw_buffer_load_b32(lat.rsrc, lat.off + (unsigned)d0 + 4u, 0, 0);
         const int up = lat.spin(st.up), dn = lat.spin(st.dn);
         const int lt_far = wl ? lat.spin(st.lt) : 0, rt_far = wr ? lat.spin(st.rt) : 0;
         // the window lo .. hi lies within the 8 bytes from d0: one 64-bit shift per byte
@@ -580,6 +585,11 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         lat.base = lat_lds + (size_t)rl * a.stride * 4;
     } else {
         lat.base = (unsigned char *)a.lattice + (size_t)(live ? r : 0) * a.stride;
+        // the workgroup's live replicas r0 .. r0 + nrep - 1 as one buffer
+        const int nrep = max(0, min(a.rpw, a.R - r0));
+        lat.rsrc = __builtin_amdgcn_make_buffer_rsrc((unsigned char *)a.lattice + (size_t)r0 * a.stride, (short)0,
+                                                     nrep * a.stride, kBufferWord3);
+        lat.off = (unsigned)(rl * a.stride);
     }
 
     // Launch-local attempt coordinates shifted by b = a_begin & 1 so that
@@ -884,6 +894,8 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     lat.base = (unsigned char *)a.lattice + (size_t)r * a.stride;
     lat.dw = nullptr;
     lat.rl = 0;
+    lat.rsrc = __builtin_amdgcn_make_buffer_rsrc(lat.base, (short)0, a.stride, kBufferWord3);
+    lat.off = 0u;
 
     const uint64_t a_begin = a.rng_dev ? a.rng_dev->attempts : a.a_begin;
     const uint64_t seed = a.rng_dev ? a.rng_dev->seed : a.seed;
