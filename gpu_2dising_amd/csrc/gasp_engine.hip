@@ -232,20 +232,20 @@ static int layout_stride(int method, int N) {
     return ((N + 15) / 16) * 16;                  // bytes per replica
 }
 
-// Entries of sweep_hbm_waves' LDS conflict table (a power of two): one per
-// site, up to what the LDS of a CU holds for the workgroups it should run at
-// once (R workgroups over the CUs, at most four per CU), and at most 16384
-// (64 KB), which leaves room on the CU for other kernels' workgroups (in a
-// batched graph, 128 KB tables kept the LDS groups off every CU the HBM group
-// held: 11.96 ms per replay against 10.38 ms).  Fewer entries than sites only
-// adds conflicts (sites share entries); at the batched L = 256 shape four
-// sites share each entry.
+// Entries of each of sweep_hbm_waves' two LDS dependence tables (a power of
+// two): one per site, up to what the LDS of a CU holds for the workgroups it
+// should run at once (R workgroups over the CUs, at most four per CU), and at
+// most 16384 (both tables 128 KB).  Fewer entries than sites only add
+// deferrals (sites share entries): at the batched L = 256 shape four sites
+// share each entry, and a 20-sweep launch there takes 3.98 ms against 4.33
+// with 8192 entries; the batched replay, whose other groups then find less
+// LDS on those CUs, still went from 5.5-5.8 to 5.4-5.5 ms.
 static int hbm_table_entries(int N, int R) {
     DeviceInfo d = device_info();
     const int ncu = d.ok ? d.ncu : 256;
     const size_t lds = d.ok ? d.lds_block : 65536;
     const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
-    const size_t budget = (lds / (size_t)per_cu - (size_t)kLdsLattice) / 4;
+    const size_t budget = (lds / (size_t)per_cu - (size_t)kLdsLattice) / 8;
     int M = 1;
     while (M < N && M < 16384 && (size_t)M * 2 <= budget) M *= 2;
     return M;
@@ -260,7 +260,7 @@ static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
     }
-    if (S > 64) return (size_t)kLdsLattice + 4 * (size_t)hbm_table_entries(N, R);  // sweep_hbm_waves table
+    if (S > 64) return (size_t)kLdsLattice + 8 * (size_t)hbm_table_entries(N, R);  // sweep_hbm_waves tables
     return (size_t)kLdsLattice;
 }
 
@@ -312,8 +312,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
     } else {
         W = g_tune_waves > 0 ? std::min(g_tune_waves, 16) : 1;
     }
-    // method 0 may spread a replica over 2, 4 or 8 waves (sweep_hbm_waves)
-    int Smax = (method == 2) ? (W >= 2 ? 32 : 16) : (method == 0 ? 512 : 64);
+    // method 0 may spread a replica over 2, 4, 8 or 16 waves (sweep_hbm_waves)
+    int Smax = (method == 2) ? (W >= 2 ? 32 : 16) : (method == 0 ? 1024 : 64);
     int S;
     if (g_tune_slots > 0) {
         S = std::min(pow2_ceil(g_tune_slots), Smax);
@@ -329,8 +329,10 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         }
         // several waves per replica only where conflicts are rare
         if (method == 0 && N < 16384) S = std::min(S, 64);
-        // eight waves per replica (512 slots) only when the tuner picks it
-        if (method == 0) S = std::min(S, 256);
+        // at most eight waves per replica: at L=256 with 256 replicas a
+        // 20-sweep launch took 5.32 ms on four waves, 3.98 on eight and 3.98
+        // on sixteen (the tuner may still pick sixteen)
+        if (method == 0) S = std::min(S, 512);
         // With replicas to spare, several slots beat one: at L=64 with 2^19
         // replicas one slot per replica (64-lane workgroups, four per CU for
         // LDS) ran at 2.61e11 attempts/s; split words with 8 slots run at
@@ -423,6 +425,7 @@ static SweepFn pick_sweep(int S) {
         if (S == 128) return sweep_hbm_waves<RNG, 2, P2>;
         if (S == 256) return sweep_hbm_waves<RNG, 4, P2>;
         if (S == 512) return sweep_hbm_waves<RNG, 8, P2>;
+        if (S == 1024) return sweep_hbm_waves<RNG, 16, P2>;
     }
     return nullptr;
 }
@@ -1600,6 +1603,7 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
         cand.push_back({128, 2});
         cand.push_back({256, 4});
         cand.push_back({512, 8});
+        cand.push_back({1024, 16});
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();

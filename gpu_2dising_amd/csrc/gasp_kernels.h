@@ -838,40 +838,38 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 }
 
 // ------------------------------------------------------------ multi-wave HBM sweep
-// Layout 0 with S = 64 W lanes per replica spread over the W waves (2, 4 or 8)
+// Layout 0 with 64 W lanes per replica spread over the W waves (2, 4, 8 or 16)
 // of one workgroup, one replica per workgroup.  For large lattices with few
 // replicas (the L = 256 group of a batched run has 256), one wave per replica
 // leaves three of each CU's four SIMDs idle and the round latency exposed.
-// Slot t = 64 k + lane of wave k evaluates the attempt pair (2t, 2t+1) of the
-// round, exactly as sweep_kernel does with S = 64 W, so the trajectory is the
-// same sequential one.
 //
-// Conflicts go through an LDS table instead of lane-to-lane compares: each
-// accepted attempt writes its order in the round (key 2t or 2t + 1) into the
-// entry of its site with an atomic minimum, and after a barrier every attempt
-// reads the entries of its five stencil sites; one below its own key means an
-// earlier accepted attempt may have changed what it read.  The table has a
-// power-of-two number of entries, and site x uses entry x & tbl_mask: when
-// the lattice has more sites than entries, sites that share an entry can only
-// add conflicts, and a spurious conflict only sends an attempt (and the rest
-// of the round) to the next round, so the trajectory is unchanged.  The round
-// commits every attempt before the first conflicting one.  The serial
-// compare chains this replaces (one readlane and ten compares per accepted
-// attempt, in the wave and then across the earlier waves' lists) were two
-// thirds of a round at the batched L = 256 shape.
-// A round has three barriers: after the table entries are written, after each
-// wave publishes its first conflicting attempt, and after the flips and the
-// table's reset, so that the next round reads both.
+// Windowed evaluation, one attempt per lane: a round evaluates a window of
+// 64 W attempts (launch-local indices) against the lattice as it was at the
+// start of the round.  Attempt j commits iff no earlier attempt of the window
+// has its site in j's stencil; otherwise it stays on its lane and is evaluated
+// again next round, before any newer attempt.  A committed attempt has read
+// what the sequential chain would have shown it: the earlier attempts it
+// depends on committed in earlier rounds, and the others neither read nor
+// wrote what it reads or writes; the window's earliest attempt always
+// commits.  The other lanes take the next attempts in wave and lane order.
+// Dependences come from an LDS table of attempt indices written with atomic
+// minima at entry site & tbl_mask (sites that share an entry only add
+// deferrals): an attempt whose five stencil entries hold an index below its
+// own has an earlier attempt of the window in its stencil.  Two tables
+// alternate by round, and each lane clears its entry of the previous round's
+// table during the next round.  Two barriers per round: after the marks,
+// and after the flips (which the next round's reads must see) and the counts
+// of deferred attempts per wave.
 template <int RNG, int W, bool POW2>
 __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
-    static_assert(W == 2 || W == 4 || W == 8, "two, four or eight waves per replica");
+    static_assert(W == 2 || W == 4 || W == 8 || W == 16, "2, 4, 8 or 16 waves per replica");
     constexpr int S = 64 * W;
     constexpr unsigned kNone = 0xffffffffu;
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
-    unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);
-    int *finv_lds = (int *)(smem + kLdsNbr);   // [W] first conflicting attempt of each wave
-    constexpr unsigned kTbl = kLdsLattice;      // LDS address of the conflict table
+    unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);  // [0]: up spins; [16 + 16 par + wave]: deferred attempts
+    const unsigned hm = a.tbl_mask;
+    const unsigned tsz = 4u * (hm + 1u);  // bytes per table; table p at kLdsLattice + p tsz
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -886,8 +884,7 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
 
     stage_thresholds(a, thr_lds, tid);
     if (tid == 0) cnt_lds[0] = 0u;
-    const unsigned hm = a.tbl_mask;
-    for (unsigned i = (unsigned)tid; i <= hm; i += S) *lds_ptr32(kTbl + 4u * i) = kNone;
+    for (unsigned i = (unsigned)tid; i < 2u * (hm + 1u); i += S) *lds_ptr32(kLdsLattice + 4u * i) = kNone;
     __syncthreads();
 
     Lattice<LAT_HBM> lat;
@@ -897,106 +894,88 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     lat.rsrc = __builtin_amdgcn_make_buffer_rsrc(lat.base, (short)0, a.stride, kBufferWord3);
     lat.off = 0u;
 
+    // attempt l of the launch is global attempt a_begin + l: Philox pair
+    // (a_begin + l) >> 1, first or second half by its low bit
     const uint64_t a_begin = a.rng_dev ? a.rng_dev->attempts : a.a_begin;
     const uint64_t seed = a.rng_dev ? a.rng_dev->seed : a.seed;
-    const uint32_t b = (uint32_t)(a_begin & 1ull);
-    const uint64_t P0 = a_begin >> 1;
-    const uint32_t e = (uint32_t)a.n_attempts + b;
-    uint32_t l0 = active ? b : e;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const Geometry<POW2> geo{a.L, a.N, a.lg};
     const uint32_t *drw = nullptr;
     if constexpr (RNG == RNG_DRAWS) drw = a.draws + (size_t)r * (3 * a.n_attempts);
-    const unsigned keyA = 2u * (unsigned)tid, keyB = keyA + 1u;
 
-    while (l0 < e) {
-        const uint32_t q0 = l0 >> 1;
-        const uint32_t q = q0 + (uint32_t)tid;
-        const uint32_t lA = 2 * q, lB = lA + 1;
-        const bool enA = (lA >= l0) && (lA < e);
-        const bool enB = (lB >= l0) && (lB < e);
-
-        Stencil sA, sB;
-        uint32_t uaA, uaB;
-        if constexpr (RNG == RNG_PHILOX) {
-            const uint64_t pair = P0 + q;
-            uint32_t c0 = (uint32_t)pair, c1 = (uint32_t)(pair >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
-            philox4x32_10(c0, c1, c2, c3, k0, k1);
-            sA = geo.philox(c0);
-            uaA = c1;
-            sB = geo.philox(c2);
-            uaB = c3;
-        } else {
-            uint32_t u0 = 0, u1 = 0, u2 = 0, v0 = 0, v1 = 0, v2 = 0;
-            if (enA) {
-                const uint32_t *d = drw + 3 * (size_t)(lA - b);
-                u0 = d[0]; u1 = d[1]; u2 = d[2];
+    const uint32_t n = active ? (uint32_t)a.n_attempts : 0u;
+    bool has = (uint32_t)tid < n;
+    uint32_t l = (uint32_t)tid;  // launch-local attempt index of this lane
+    uint32_t next = min((uint32_t)S, n);
+    unsigned par = 0u, prev_ent = 0u;
+    bool had_prev = false;
+    for (;;) {
+        Stencil st{};
+        bool acc = false;
+        int s01 = 0;
+        if (has) {
+            uint32_t ua;
+            if constexpr (RNG == RNG_PHILOX) {
+                const uint64_t A = a_begin + (uint64_t)l;
+                const uint64_t pair = A >> 1;
+                uint32_t c0 = (uint32_t)pair, c1 = (uint32_t)(pair >> 32), c2 = (uint32_t)r, c3 = kPhiloxDomain;
+                philox4x32_10(c0, c1, c2, c3, k0, k1);
+                const bool second = (A & 1ull) != 0ull;
+                st = geo.philox(second ? c2 : c0);
+                ua = second ? c3 : c1;
+            } else {
+                const uint32_t *d = drw + 3 * (size_t)l;
+                st = geo.mt(d[0], d[1]);
+                ua = d[2];
             }
-            if (enB) {
-                const uint32_t *d = drw + 3 * (size_t)(lB - b);
-                v0 = d[0]; v1 = d[1]; v2 = d[2];
-            }
-            sA = geo.mt(u0, u1);
-            uaA = u2;
-            sB = geo.mt(v0, v1);
-            uaB = v2;
+            __hip_atomic_fetch_min(lds_ptr32(kLdsLattice + par * tsz + 4u * ((unsigned)st.c & hm)), l,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const Spins x = read_spins<LAT_HBM>(lat, st);
+            const Attempt t = decide(st, x, ua);
+            acc = t.acc;
+            s01 = t.s01;
         }
-        const Spins xA = read_spins<LAT_HBM>(lat, sA), xB = read_spins<LAT_HBM>(lat, sB);
-        const Attempt A = decide(sA, xA, uaA), B = decide(sB, xB, uaB);
-        const bool accA = A.acc && enA, accB = B.acc && enB;
-
-        // mark: the lowest key of the round's accepted attempts per entry
-        if (accA) __hip_atomic_fetch_min(lds_ptr32(kTbl + 4u * ((unsigned)A.st.c & hm)), keyA,
-                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (accB) __hip_atomic_fetch_min(lds_ptr32(kTbl + 4u * ((unsigned)B.st.c & hm)), keyB,
-                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // the previous round's mark, in the other table (read by nobody now)
+        if (had_prev) *lds_ptr32(kLdsLattice + (par ^ 1u) * tsz + 4u * prev_ent) = kNone;
         __syncthreads();
-        // test: an entry of the stencil below the attempt's own key (the
-        // attempt's own mark is its key, not below it)
-        const unsigned mAx = min(min(min(lds_load32(kTbl + 4u * ((unsigned)A.st.c & hm)),
-                                         lds_load32(kTbl + 4u * ((unsigned)A.st.up & hm))),
-                                     min(lds_load32(kTbl + 4u * ((unsigned)A.st.dn & hm)),
-                                         lds_load32(kTbl + 4u * ((unsigned)A.st.lt & hm)))),
-                                 lds_load32(kTbl + 4u * ((unsigned)A.st.rt & hm)));
-        const unsigned mBx = min(min(min(lds_load32(kTbl + 4u * ((unsigned)B.st.c & hm)),
-                                         lds_load32(kTbl + 4u * ((unsigned)B.st.up & hm))),
-                                     min(lds_load32(kTbl + 4u * ((unsigned)B.st.dn & hm)),
-                                         lds_load32(kTbl + 4u * ((unsigned)B.st.lt & hm)))),
-                                 lds_load32(kTbl + 4u * ((unsigned)B.st.rt & hm)));
-        // this wave's first conflicting enabled attempt (local index 2 lane + (B ? 1 : 0); 128 = none)
-        const uint64_t iA = wave_mask(mAx < keyA) & wave_mask(enA);
-        const uint64_t iB = wave_mask(mBx < keyB) & wave_mask(enB);
-        const uint64_t x = iA | iB;
-        if (lane == 0) {
-            int f = 128;
-            if (x != 0ull) {
-                const int j = __builtin_ctzll(x);
-                f = 2 * j + (((iA >> j) & 1ull) ? 0 : 1);
-            }
-            finv_lds[wave] = f;
+        bool deferred = false;
+        if (has) {
+            const unsigned tb = kLdsLattice + par * tsz;
+            const uint32_t m = min(min(min(lds_load32(tb + 4u * ((unsigned)st.c & hm)),
+                                           lds_load32(tb + 4u * ((unsigned)st.up & hm))),
+                                       min(lds_load32(tb + 4u * ((unsigned)st.dn & hm)),
+                                           lds_load32(tb + 4u * ((unsigned)st.lt & hm)))),
+                                   lds_load32(tb + 4u * ((unsigned)st.rt & hm)));
+            deferred = m < l;
+            if (!deferred && acc) lat.flip(st.c, s01);
         }
+        const uint64_t dmask = wave_mask(deferred);
+        if (lane == 0) cnt_lds[16u + 16u * par + (unsigned)wave] = (uint32_t)__popcll(dmask);
+        had_prev = has;
+        prev_ent = (unsigned)st.c & hm;
+        // flips visible to every wave before the next round's reads
         __syncthreads();
-        // the round's first conflicting attempt, as an offset from attempt 2 q0
-        int F = 2 * S;
-        for (int k = W - 1; k >= 0; --k) {
-            const int f = __builtin_amdgcn_readfirstlane(finv_lds[k]);
-            if (f < 128) F = 128 * k + f;
+        // deferred attempts keep their lanes; the others take the next new
+        // attempts in wave order, lane order
+        uint32_t D = 0u, before = 0u;
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+            const uint32_t dv = cnt_lds[16u + 16u * par + (unsigned)v];
+            D += dv;
+            if (v < wave) before += 64u - dv;
         }
-        const int tA = 128 * wave + 2 * lane;
-        if (accA && tA < F) lat.flip(A.st.c, A.s01);
-        if (accB && tA + 1 < F) lat.flip(B.st.c, B.s01);
-        // every reader has read: clear what this round marked
-        if (accA) *lds_ptr32(kTbl + 4u * ((unsigned)A.st.c & hm)) = kNone;
-        if (accB) *lds_ptr32(kTbl + 4u * ((unsigned)B.st.c & hm)) = kNone;
-        if (F < 2 * S) {
-            l0 = 2 * q0 + (uint32_t)F;
-        } else {
-            const uint32_t nxt = 2 * (q0 + S);
-            l0 = nxt < e ? nxt : e;
+        const uint32_t rem = n - next;
+        if (D == 0u && rem == 0u) break;
+        if (!deferred) {
+            const uint64_t free_mask = ~dmask;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(free_mask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)free_mask, 0u));
+            const uint32_t idx = before + rank;
+            has = idx < rem;
+            l = next + idx;
         }
-        // flips visible to every wave, and the table clear, before the next
-        // round's reads and marks
-        __syncthreads();
+        next += min((uint32_t)S - D, rem);
+        par ^= 1u;
     }
 
     // ---------------------------------------------------------- epilogue

@@ -179,6 +179,6 @@ def test_autotune_picks_valid_shape(gpu, engine_lib):
     print("[autotune]", flush=True)
     for method, L, R in [(2, 32, 1024), (1, 64, 512), (0, 64, 64)]:
         s, w, k, rate = G.autotune(method, L, R, 2, 0.55, 0.05, 20)
-        assert s in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512) and w in (1, 2, 4, 8) and rate > 0
+        assert s in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) and w in (1, 2, 4, 8, 16) and rate > 0
         assert k in (20, 10, 5) and 20 % k == 0
         assert G.get_tuning(method, L, R) == (s, w)

@@ -18,10 +18,10 @@ pytestmark = pytest.mark.gpu
 BETA, H = 0.54, 0.07
 # (method, slots per replica S, second tuning value: waves for 0/1, word split for 2:
 # 1, 2, 4, 8 = 64, 32, 16, 8 replicas per workgroup)
-# Method 0 with 128 / 256 / 512 slots spreads a replica over 2 / 4 / 8 waves (sweep_hbm_waves).
+# Method 0 with 128 / 256 / 512 / 1024 slots spreads a replica over 2 / 4 / 8 / 16 waves (sweep_hbm_waves).
 CASES = ([(2, s, 1) for s in (1, 2, 4, 8, 16)] + [(2, s, 2) for s in (1, 4, 16, 32)] +
          [(2, s, 4) for s in (1, 8, 32)] + [(2, s, 8) for s in (2, 16, 32)] +
-         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4), (0, 512, 8)])
+         [(1, s, 1) for s in (1, 2, 8, 64)] + [(0, s, 1) for s in (1, 4, 64)] + [(0, 128, 2), (0, 256, 4), (0, 512, 8), (0, 1024, 16)])
 
 
 def _start(kind, R, L, seed):
@@ -102,7 +102,7 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
     assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
-@pytest.mark.parametrize("S", [128, 256, 512])
+@pytest.mark.parametrize("S", [128, 256, 512, 1024])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
     """A replica spread over 2, 4 or 8 waves at the batched run's L = 256
     shape (R = 6, so the conflict table has fewer entries than sites and
@@ -120,10 +120,10 @@ def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
 
 
 def test_hbm_waves_small_table(gpu, engine_lib, oracle):
-    """600 replicas of L = 256 over four waves each: the conflict table is
-    sized for three workgroups per CU (8192 entries on a 160 KB-LDS device),
-    so eight sites share each entry and spurious conflicts are common, which
-    must only delay attempts, never change them."""
+    """600 replicas of L = 256 over four waves each: the dependence tables
+    are sized for three workgroups per CU (4096 entries each on a 160 KB-LDS
+    device), so sixteen sites share each entry and spurious dependences are
+    common, which must only delay attempts, never change them."""
     G, O = engine_lib, oracle
     L, R, seed = 256, 600, 777
     for start, beta in (("down", BETA), ("random", 0.3)):

@@ -231,7 +231,7 @@ print("OK")
 
 def test_default_launch_shapes(engine_lib):
     """Launch shapes chosen without the tuner (256 CUs): method 0 runs a whole
-    wave per replica from N >= 4096, and 2 or 4 waves per replica (8 only from the tuner) when
+    wave per replica from N >= 4096, and 2, 4 or 8 waves per replica (16 only from the tuner) when
     N >= 16384 and the replicas alone would not fill the device; method 2 keeps
     split words with 16 slots per replica at L=64 with 16384 replicas, splits
     further (16 or 8 replicas per workgroup) when halves would leave CUs
@@ -242,7 +242,8 @@ def test_default_launch_shapes(engine_lib):
     try:
         assert G.get_tuning(0, 64, 16384) == (64, 1)
         assert G.get_tuning(0, 64, 256) == (64, 1)
-        assert G.get_tuning(0, 256, 256) == (256, 4)
+        assert G.get_tuning(0, 256, 256) == (512, 8)
+        assert G.get_tuning(0, 256, 1024) == (256, 4)
         assert G.get_tuning(0, 256, 2048) == (128, 2)
         assert G.get_tuning(0, 256, 16384) == (64, 1)
         assert G.get_tuning(2, 64, 16384) == (16, 2)  # split words
