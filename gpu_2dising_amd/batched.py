@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import CHECK_COUNT, CHECK_FATES, Engine, Graph, RngState, autotune
+from . import CHECK_COUNT, CHECK_FATES, Engine, Graph, RngState, autotune, set_tuning
 
 # L, replicas, gpu_method: four lattice sizes on the LDS multispin path and
 # one on the HBM byte lattice
@@ -56,21 +56,33 @@ class Batch:
     graph is `chk` sweeps of every group plus its threshold check."""
 
     def __init__(self, groups=CONFIG4_GROUPS, chk: int = 20, seed: int = 2026, tune: bool = True,
-                 probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9):
+                 probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9, shapes=None):
+        """`shapes`: optional (slots, waves, sweeps per launch) per group, used
+        instead of the tuner's choice (for comparing builds at fixed shapes)."""
         rs = np.random.default_rng(seed)
         self.chk, self.dn, self.up = chk, dn, up
         self.groups: list[Group] = []
         for k, (L, R, m) in enumerate(groups):
             beta, h = float(rs.uniform(0.50, 0.60)), float(rs.uniform(0.03, 0.10))
             g = Group(L=L, R=R, method=m, beta=beta, h=h, itask=k % 2, seed=seed + k)
-            if tune:
+            if shapes is not None:
+                g.slots, g.waves, g.launch_sweeps = shapes[k]
+            elif tune:
                 g.slots, g.waves, g.launch_sweeps, g.probe_rate = autotune(m, L, R, probe_sweeps, beta, h, chk)
             else:
                 g.launch_sweeps = chk
             self.groups.append(g)
         for g in self.groups:
             g.rng = RngState(g.R, g.seed)
-            g.engine = Engine(g.L, g.R, g.method, g.rng, g.beta, g.h)
+            if shapes is not None:
+                set_tuning(g.slots, g.waves)
+            try:
+                g.engine = Engine(g.L, g.R, g.method, g.rng, g.beta, g.h)
+            finally:
+                if shapes is not None:
+                    set_tuning(0, 0)
+            if shapes is not None:
+                g.probe_rate = 1.0  # capture order by the groups' attempts
             if g.itask == 0:
                 g.start = -np.ones((g.R, g.L, g.L), dtype=np.int32)
                 g.engine.set_grids(g.start)
