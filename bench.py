@@ -58,19 +58,20 @@ def committor_inputs(n: int, L: int = 64, seed: int = 12345) -> np.ndarray:
 
 
 def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 20000,
-                  chk: int = 100, time_cap: float = 240.0) -> dict:
+                  chk: int = 100, time_cap: float = 240.0, method: int = 2) -> dict:
     """Committor shooting (itask 1): every input grid gets `per` replicas with
     independent Philox streams, run until all fates are decided against
     dn = -0.90 / up = +0.90 (the CLI thresholds) or max_sweeps; replicas with a
     decided fate stop sweeping.  pB per input and its bootstrap error
     (bootstrap_errbar's estimator on glibc's rand() sequence, as the
-    reference, on host threads) are computed on the host."""
+    reference, on host threads) are computed on the host.  method 3: the
+    bit-sliced engine (the 64 replicas of a word share their sites)."""
     import ctypes as C
 
     inputs = committor_inputs(n_inputs, L)
     R_c = n_inputs * per
     rng = G.RngState(R_c, 777)
-    e = G.Engine(L, R_c, 2, rng, BETA, H)
+    e = G.Engine(L, R_c, method, rng, BETA, H)
     e.set_inputs(inputs)
     e.set_freeze(True)
     e.set_fates(np.full(R_c, -1, dtype=np.int32))
@@ -127,7 +128,45 @@ def committor_leg(G, n_inputs: int = 512, per: int = 1024, max_sweeps: int = 200
         "workload": "512 grids: -1 sea, +1 disc of radius U[4,14] at a random centre, 5% of spins "
                     "flipped; 1024 replicas each; thresholds -0.90/+0.90; <= 20000 sweeps",
         "_pB": pB,
+        "_fates": f,
     }
+
+
+def committor_sliced_leg(G, shooting: dict) -> dict:
+    """The committor leg on the bit-sliced engine (method 3): the same 512
+    inputs x 1024 replicas, with the replicas of each 64-replica word sharing
+    their site sequence.  Reported against the per-replica-stream shooting:
+    the MAE of the two pB estimates (two independent 1024-replica estimates
+    differ by about sqrt(2 pB (1 - pB) / 1024) each), and the dispersion of
+    the 16 word means of each input against the binomial variance pB (1 - pB)
+    / 64 (1 when the replicas of a word are independent; inputs with pB in
+    [0.05, 0.95])."""
+    per = shooting["replicas_per_input"]
+    out = committor_leg(G, per=per, method=3)
+    pB, ref = out.pop("_pB"), shooting["_pB"]
+    ok = np.isfinite(pB) & np.isfinite(ref)
+    out["metric"] = "committor pB estimates/sec (shooting, bit-sliced engine)"
+    out["mae_vs_per_replica_shooting"] = float(np.mean(np.abs(pB[ok] - ref[ok])))
+    pm = 0.5 * (pB[ok] + ref[ok])
+    out["expected_mae_two_independent_estimates"] = float(np.mean(np.sqrt(2.0 * pm * (1.0 - pm) / per)) * np.sqrt(2.0 / np.pi))
+    out["speedup_vs_per_replica_shooting"] = out["value"] / shooting["value"]
+    f = out.pop("_fates")
+    words = (f == 1).reshape(f.shape[0], per // 64, 64).mean(2)  # [input][word]
+    p = (f == 1).mean(1)
+    sel = (p > 0.05) & (p < 0.95) & (f >= 0).all(1)
+    disp = words[sel].var(1, ddof=1) / (p[sel] * (1.0 - p[sel]) / 64.0)
+    out["word_dispersion"] = {"mean": float(disp.mean()), "inputs": int(sel.sum()),
+                              "note": "variance of the 16 word means of an input over p(1-p)/64; 1 = independent replicas"}
+    # the same statistic on the per-replica streams (independent by construction)
+    fr = shooting["_fates"]
+    wr = (fr == 1).reshape(fr.shape[0], per // 64, 64).mean(2)
+    pr = (fr == 1).mean(1)
+    sr = (pr > 0.05) & (pr < 0.95) & (fr >= 0).all(1)
+    out["word_dispersion_per_replica_streams"] = float((wr[sr].var(1, ddof=1) / (pr[sr] * (1.0 - pr[sr]) / 64.0)).mean())
+    # correlated replicas inflate an estimate's variance by the dispersion (the
+    # design effect), so equal error needs that many times the replicas
+    out["speedup_at_equal_error"] = out["speedup_vs_per_replica_shooting"] / max(out["word_dispersion"]["mean"], 1.0)
+    return out
 
 
 SURROGATE_PARAMS = os.path.join(ROOT, "gpu_2dising_amd", "surrogate_L64.npz")
@@ -541,6 +580,8 @@ def main() -> int:
         out["sliced"] = sliced_leg(G, args.steps, args.warmup)
     if not args.no_committor:
         out["committor"] = committor_leg(G)
+        if not args.no_sliced:
+            out["committor_sliced"] = committor_sliced_leg(G, out["committor"])
         if not args.no_surrogate and (os.path.exists(SURROGATE_PARAMS) or os.path.exists(SURROGATE_PARAMS_TRAJ)):
             out["surrogate"] = surrogate_leg(G, out["committor"])
     if not args.no_batched:
@@ -564,6 +605,7 @@ def main() -> int:
                 cm["gpu_over_cpu_1024"] = cm["value"] / cc["value_1024_upper_bound"]
     if "committor" in out:
         out["committor"].pop("_pB")
+        out["committor"].pop("_fates")
     print(json.dumps(out), flush=True)
     return 0
 
