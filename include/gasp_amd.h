@@ -216,9 +216,9 @@ void gasp_free(void *d_ptr);
  * 8 = a 16- or 8-bit part (16 or 8 replicas, slots up to 32); 0 = automatic:
  * halves, or smaller parts when that leaves fewer workgroups than CUs or
  * when the lattice would not fit the LDS otherwise.
- * Method 0 also takes 128, 256 or 512 slots: one replica per workgroup, spread
- * over 2, 4 or 8 waves (the waves value is then implied).  Methods 1 and 2: up
- * to 64.  Method 3 (bit-sliced, below): the first value is the number of
+ * Method 0 also takes 128, 256, 512 or 1024 slots: one replica per workgroup,
+ * spread over 2, 4, 8 or 16 waves (the waves value is then implied).  Methods
+ * 1 and 2: up to 64.  Method 3 (bit-sliced, below): the first value is the number of
  * waves per workgroup (1, 2, 4 or 8) that evaluate one word group's attempts
  * together, the second the word split (1: 64 replicas per word, 2: 32).
  * Applies to subsequent runs. */
