@@ -35,7 +35,8 @@ EXPORTS = (
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
     "gasp_engine_sweep", "gasp_engine_sync", "gasp_engine_get_magnetisation",
     "gasp_engine_get_counts", "gasp_engine_set_fates", "gasp_engine_get_fates",
-    "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream", "gasp_sweep_grids",
+    "gasp_engine_largest_cluster", "gasp_engine_kernel_ms", "gasp_engine_stream", "gasp_engine_get_shape",
+    "gasp_sweep_grids",
     "gasp_sweep_grids_noalloc",
     "gasp_graph_create", "gasp_graph_launch", "gasp_graph_ms", "gasp_graph_destroy", "gasp_autotune",
     "gasp_bootstrap_errbar_gpu", "gasp_bootstrap_errbar_libc",
@@ -155,6 +156,7 @@ def lib() -> C.CDLL:
     L.gasp_engine_largest_cluster.argtypes = [vp, C.c_int, fp]
     L.gasp_engine_kernel_ms.argtypes = [vp]
     L.gasp_engine_kernel_ms.restype = C.c_double
+    L.gasp_engine_get_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
     L.gasp_sweep_grids.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
@@ -309,6 +311,12 @@ class Engine:
 
     def kernel_ms(self) -> float:
         return lib().gasp_engine_kernel_ms(self.ptr)
+
+    def shape(self) -> tuple:
+        """(slots per replica, waves per workgroup or word split) of the engine's launches."""
+        s, w = C.c_int(0), C.c_int(0)
+        check(lib().gasp_engine_get_shape(self.ptr, C.byref(s), C.byref(w)))
+        return s.value, w.value
 
     def close(self) -> None:
         if self.ptr:

@@ -218,7 +218,7 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
 
 static int pow2_ceil(long long x) {
     int p = 1;
-    while (p < x && p < 512) p <<= 1;
+    while (p < x && p < 1024) p <<= 1;
     return p;
 }
 
@@ -1138,6 +1138,13 @@ extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
 }
 
 extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
+
+extern "C" int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves) {
+    if (!e || !slots || !waves) return set_error(GASP_ERR_ARG, "gasp_engine_get_shape: NULL argument");
+    *slots = e->S;
+    *waves = e->W;
+    return GASP_OK;
+}
 
 // ============================================================ caller grids
 // Stream-ordered sweeps of caller-owned int32 device grids (the layout

@@ -329,6 +329,9 @@ int gasp_sweep_grids_noalloc(int L, int ngrids, int gpu_method, gasp_rng_state *
 double gasp_engine_kernel_ms(gasp_engine *e);
 /* The engine's HIP stream (hipStream_t). */
 void *gasp_engine_stream(gasp_engine *e);
+/* The engine's launch shape: slots per replica and waves per workgroup (the
+ * word split for method 2), as gasp_set_tuning takes them. */
+int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves);
 
 /* ------------------------------------------------ batched graph, autotuner */
 /* One captured hipGraph advancing n engines concurrently (uneven L, methods,

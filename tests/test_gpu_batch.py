@@ -176,8 +176,8 @@ def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
 
 def test_batched_fixed_shapes(gpu, engine_lib, oracle):
     """Batch with explicit (slots, waves, sweeps per launch) per group instead
-    of the tuner's: each engine runs at the given shape (the run statistics of
-    a direct sweep report it), and its replicas follow the oracle."""
+    of the tuner's: each engine runs at the given shape, and its replicas
+    follow the oracle."""
     from gpu_2dising_amd.batched import Batch
 
     G = engine_lib
@@ -187,10 +187,9 @@ def test_batched_fixed_shapes(gpu, engine_lib, oracle):
     rs = np.random.default_rng(9)
     try:
         for g, (s_, w_, _) in zip(b.groups, shapes):
+            assert g.engine.shape() == (s_, w_), (g.L, g.engine.shape())
             g.engine.sweep(b.chk, G.CHECK_COUNT if g.itask == 0 else G.CHECK_FATES, b.dn, b.up)
             g.engine.sync()
-            st = G.run_stats()
-            assert (st.slots_per_replica, st.waves_per_group) == (s_, w_), (g.L, st.slots_per_replica, st.waves_per_group)
             idx = np.unique(np.concatenate([[0, g.R - 1], rs.choice(g.R, 6, replace=False)]))
             grids = g.start[idx].copy()
             oracle.sweep_philox_replicas(g.L, grids, idx, g.seed, 0, b.chk * g.L * g.L, g.beta, g.h)
