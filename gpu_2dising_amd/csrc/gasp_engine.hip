@@ -330,8 +330,8 @@ static void choose_tuning(int method, int L, int R, int *S_out, int *W_out) {
         // several waves per replica only where conflicts are rare
         if (method == 0 && N < 16384) S = std::min(S, 64);
         // at most eight waves per replica: at L=256 with 256 replicas a
-        // 20-sweep launch took 5.32 ms on four waves, 3.98 on eight and 3.98
-        // on sixteen (the tuner may still pick sixteen)
+        // 20-sweep launch took 5.32 ms on four waves and 3.97 on eight; sixteen
+        // (3.41 ms alone) crowd other kernels' workgroups off the CUs
         if (method == 0) S = std::min(S, 512);
         // With replicas to spare, several slots beat one: at L=64 with 2^19
         // replicas one slot per replica (64-lane workgroups, four per CU for
@@ -1610,7 +1610,10 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
         cand.push_back({128, 2});
         cand.push_back({256, 4});
         cand.push_back({512, 8});
-        cand.push_back({1024, 16});
+        // not 1024 (sixteen waves): alone, the batched L = 256 group's 20 sweeps
+        // take 3.41 ms against 3.97 on eight waves, but in the batched graph
+        // its 1024-lane workgroups crowd the other groups' (7.0-7.2 ms per
+        // replay against 5.35-5.38), and the tuner probes a group alone
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();
