@@ -413,7 +413,8 @@ def throughput_run(G, method: int, steps: int, warmup: int, dump: bool):
     steps, a restart from all -1, then `steps` timed steps of one 100-sweep
     launch with the fused check and the host reading the count, a bit-packed
     gridstates.bin record to host memory every 10th step.  Returns the wall
-    time, the sweep kernel's time (HIP events) and the nucleated fractions."""
+    time, the sweep kernel's time (HIP events), the nucleated fractions and
+    the engine's conflict filter (1 row/column masks, 2 mark plane)."""
     rng = G.RngState(R, 20251202)
     eng = G.Engine(L, R, method, rng, BETA, H)
     g0 = -np.ones((R, L, L), dtype=np.int32)
@@ -436,9 +437,10 @@ def throughput_run(G, method: int, steps: int, warmup: int, dump: bool):
     eng.sync()
     wall = time.perf_counter() - t0
     kern_ms = eng.kernel_ms() - ms0
+    filt = eng.conflict_filter()
     eng.close()
     rng.free()
-    return wall, kern_ms, frac
+    return wall, kern_ms, frac, filt
 
 
 def sliced_leg(G, steps: int, warmup: int) -> dict:
@@ -447,7 +449,7 @@ def sliced_leg(G, steps: int, warmup: int) -> dict:
     bitwise adders, acceptance by bit-sliced comparison of per-replica draws),
     at the tuner's launch shape."""
     S3, split3 = G.get_tuning(3, L, R)
-    wall, kern_ms, frac = throughput_run(G, 3, steps, warmup, True)
+    wall, kern_ms, frac, _ = throughput_run(G, 3, steps, warmup, True)
     attempts = float(steps) * SWEEPS_PER_STEP * L * L * R
     per_launch_ms = kern_ms / max(steps, 1)
     wb = 64 // split3
@@ -494,7 +496,7 @@ def main() -> int:
     G.init_device(-1)
     G.set_tuning(args.slots, args.word_split)
     S, split = G.get_tuning(args.method, L, R)
-    wall, kern_ms, frac = throughput_run(G, args.method, args.steps, args.warmup, not args.no_dump)
+    wall, kern_ms, frac, filt = throughput_run(G, args.method, args.steps, args.warmup, not args.no_dump)
 
     attempts = float(args.steps) * SWEEPS_PER_STEP * L * L * R
     value = attempts / wall
@@ -534,6 +536,7 @@ def main() -> int:
             "total_sweeps": args.steps * SWEEPS_PER_STEP, "gpu_method": args.method,
             "rng": "philox4x32-10 per replica", "slots_per_replica": S,
             "replicas_per_workgroup": 64 // max(split, 1) if args.method == 2 else None,
+            "conflict_filter": ({1: "row/column masks", 2: "mark plane"}[filt] if args.method == 2 else None),
             "grid_dumps": not args.no_dump,
         },
         "roofline": {
@@ -545,7 +548,7 @@ def main() -> int:
             "traffic": None,
             "bytes_per_attempt": bpa,
             "kernel": (f"sweep_sliced<{64 // max(split, 1)}, {S}, true>" if args.method == 3
-                       else f"sweep_kernel<{kname}, RNG_PHILOX, S>"),
+                       else f"sweep_kernel<{kname}, RNG_PHILOX, S{', MOFF' if filt == 2 else ''}>"),
             "kernel_ms_per_launch": per_launch_ms,
             "kernel_attempts_per_s": kern_rate,
         },
@@ -573,8 +576,15 @@ def main() -> int:
             out["roofline"]["valu"] = {"instructions_per_attempt": vpa, "achieved": kern_rate * vpa,
                                        "peak": VALU_PEAK, "unit": "wave64 instr/s",
                                        "frac": kern_rate * vpa / VALU_PEAK}
-            # the binding resource is the higher of the two fractions
-            out["roofline"]["limiter"] = "valu" if kern_rate * vpa / VALU_PEAK > achieved / peak else "lds"
+            # the LDS pipe's busy share (SQ_LDS_IDX_ACTIVE over the kernel's cycles, bank
+            # conflicts included) from the same counter file
+            busy = prof.get("lds_active_share")
+            if busy:
+                out["roofline"]["lds_pipe"] = {"active_share": busy,
+                                               "bank_conflict_share": prof.get("lds_bank_conflict_share")}
+            # the binding resource is the highest of the fractions
+            fr = {"valu": kern_rate * vpa / VALU_PEAK, "lds bandwidth": achieved / peak, "lds pipe": busy or 0.0}
+            out["roofline"]["limiter"] = max(fr, key=fr.get)
     G.set_tuning(0, 0)
     if not args.no_sliced and args.method != 3:
         out["sliced"] = sliced_leg(G, args.steps, args.warmup)

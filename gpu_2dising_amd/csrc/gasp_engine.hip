@@ -97,6 +97,7 @@ uint64_t g_thr_f32[10];
 
 int g_tune_slots = 0;
 int g_tune_waves = 0;
+int g_tune_filter = 0;  // method 2 on half words: 1 row/column masks, 2 mark plane (0: automatic)
 thread_local gasp_run_stats_t t_stats{};
 
 // Thresholds in kernel order j = 5*s01 + (number of up neighbours).  With
@@ -215,6 +216,8 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
     g_tune_slots = slots_per_replica;
     g_tune_waves = waves_per_group;
 }
+
+extern "C" void gasp_set_conflict_filter(int filter) { g_tune_filter = filter; }
 
 static int pow2_ceil(long long x) {
     int p = 1;
@@ -405,8 +408,26 @@ extern "C" void gasp_get_run_stats(gasp_run_stats_t *out) {
 // ============================================================ kernel table
 using SweepFn = void (*)(SweepArgs);
 
+template <int LAT, int RNG, bool P2, int MOFF>
+static SweepFn pick_marks(int S) {
+    switch (S) {
+        case 2: return sweep_kernel<LAT, RNG, 2, P2, MOFF>;
+        case 4: return sweep_kernel<LAT, RNG, 4, P2, MOFF>;
+        case 8: return sweep_kernel<LAT, RNG, 8, P2, MOFF>;
+        case 16: return sweep_kernel<LAT, RNG, 16, P2, MOFF>;
+        case 32: return sweep_kernel<LAT, RNG, 32, P2, MOFF>;
+        default: return nullptr;
+    }
+}
+
+// moff: 0 (row/column filter) or the mark plane's offset above the half-word
+// lattice (16384 or 32768 bytes).
 template <int LAT, int RNG, bool P2>
-static SweepFn pick_sweep(int S) {
+static SweepFn pick_sweep(int S, int moff) {
+    if constexpr (LAT == LAT_MULTIHALF) {
+        if (moff == 16384) return pick_marks<LAT, RNG, P2, 16384>(S);
+        if (moff == 32768) return pick_marks<LAT, RNG, P2, 32768>(S);
+    }
     switch (S) {
         case 1: return sweep_kernel<LAT, RNG, 1, P2>;
         case 2: return sweep_kernel<LAT, RNG, 2, P2>;
@@ -431,8 +452,8 @@ static SweepFn pick_sweep(int S) {
 }
 
 template <int LAT, int RNG>
-static SweepFn pick_sweep2(int S, bool pow2) {
-    return pow2 ? pick_sweep<LAT, RNG, true>(S) : pick_sweep<LAT, RNG, false>(S);
+static SweepFn pick_sweep2(int S, bool pow2, int moff = 0) {
+    return pow2 ? pick_sweep<LAT, RNG, true>(S, moff) : pick_sweep<LAT, RNG, false>(S, moff);
 }
 
 template <int WB, bool P2>
@@ -452,10 +473,11 @@ static SweepFn sliced_fn(int waves, int split, bool pow2) {
     return pow2 ? pick_sliced<64, true>(waves) : pick_sliced<64, false>(waves);
 }
 
-static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2) {
+static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int moff = 0) {
     if (method == 3) return rng == 0 ? sliced_fn(S, W, pow2) : nullptr;
     if (method == 2 && W == 2)
-        return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2);
+        return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2, moff)
+                        : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2, moff);
     if (method == 2 && W == 4)
         return rng == 0 ? pick_sweep2<LAT_MULTI16, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI16, RNG_DRAWS>(S, pow2);
     if (method == 2 && W == 8)
@@ -533,6 +555,35 @@ static int launch_shape(int method, int L, int R, LaunchShape *sh) {
     return GASP_OK;
 }
 
+static size_t marks_lds_bytes(int moff, int stride) {
+    return (size_t)kLdsLattice + (size_t)moff + 4 * (size_t)stride;
+}
+
+// Conflict filter of method 2 on half words (W = 2) with 2 to 32 slots: the
+// mark plane (returns its offset above the lattice) where a second
+// lattice-sized plane fits below the 64 KB reach of a ds instruction's
+// offset (4 N <= 32768, L <= 90) and its LDS leaves as many workgroups
+// resident per CU as the launch can use; else 0, the row/column masks.  At
+// L = 64 a 100-sweep launch of 16384 replicas (two workgroups per CU) takes
+// 17.85 ms with the marks against 21.28 ms, but one of 524288 replicas takes
+// 453 ms against 412: there the doubled LDS halves the resident workgroups.
+static int choose_filter(int method, const LaunchShape &sh) {
+    if (method != 2 || sh.W != 2 || sh.S < 2 || sh.S > 32 || g_tune_filter == 1) return 0;
+    if (4 * sh.stride > 32768) return 0;
+    const int moff = 4 * sh.stride <= 16384 ? 16384 : 32768;
+    if (g_tune_filter == 2) return moff;
+    DeviceInfo d = device_info();
+    const size_t lds_cu = d.ok ? d.lds_block : 65536;
+    const int waves = std::max(1, (sh.block + 63) / 64);
+    auto resident = [&](size_t lds) {
+        return std::max<long long>(1, std::min<long long>((long long)(lds_cu / lds), 32 / waves));
+    };
+    const long long need = (sh.nblocks + sh.ncu - 1) / std::max(sh.ncu, 1);
+    const long long plain = std::min(need, resident(sh.lds));
+    const long long marks = std::min(need, resident(marks_lds_bytes(moff, sh.stride)));
+    return marks >= plain ? moff : 0;
+}
+
 // ============================================================ engine object
 struct gasp_engine {
     int L = 0, N = 0, R = 0, method = 0;
@@ -551,6 +602,7 @@ struct gasp_engine {
     size_t tmp32_bytes = 0;
     int freeze = 0;          // skip replicas whose fate is decided
     int S0 = 1;              // slots per replica chosen at creation
+    int moff = 0;            // mark plane offset above the lattice (0: row/column filter)
     bool adapt = false;      // raise the slots as replicas are decided (see engine_adapt_slots)
     int decided = 0;         // decided replicas at the last counts read (or set_fates)
     int ncu = 256;
@@ -641,12 +693,15 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->stride = sh.stride;
     e->ncu = sh.ncu;
     e->lds = sh.lds;
+    // the mark plane (if any) lies above the lattice, in the same allocation
+    e->moff = choose_filter(method, sh);
+    if (e->moff) e->lds = marks_lds_bytes(e->moff, e->stride);
     e->tbl = sh.tbl;
     e->rpw = sh.rpw;
     e->block = sh.block;
     e->nblocks = sh.nblocks;
     e->lat_bytes = sh.lat_bytes;
-    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L));
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L), e->moff);
     if (!fn) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
@@ -723,7 +778,7 @@ static int engine_adapt_slots(gasp_engine *e) {
     const int smax = 16;
     const int S = std::max(std::min(pow2_ceil((target + active - 1) / active), smax), e->S0);
     if (S == e->S) return GASP_OK;
-    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, S, e->W, is_pow2(e->L));
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, S, e->W, is_pow2(e->L), e->moff);
     if (!fn) return GASP_OK;
     if (e->lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds));
@@ -762,7 +817,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.tbl_mask = (unsigned)(e->tbl - 1);
     a.dn_thr = dn;
     a.up_thr = up;
-    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L));
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L), e->moff);
     if (check != CHECK_NONE) {
         hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
         HIP_TRY(hipGetLastError());
@@ -1143,6 +1198,12 @@ extern "C" int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves) {
     if (!e || !slots || !waves) return set_error(GASP_ERR_ARG, "gasp_engine_get_shape: NULL argument");
     *slots = e->S;
     *waves = e->W;
+    return GASP_OK;
+}
+
+extern "C" int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter) {
+    if (!e || !filter) return set_error(GASP_ERR_ARG, "gasp_engine_get_conflict_filter: NULL argument");
+    *filter = e->moff ? 2 : 1;
     return GASP_OK;
 }
 

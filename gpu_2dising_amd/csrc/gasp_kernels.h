@@ -493,9 +493,45 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
     }
 }
 
-template <int LAT, int RNG, int S, bool POW2>
+// ------------------------------------------------------------ mark plane
+// An exact candidate test for the speculation groups of the split-word layout
+// (a dword per site, bit rl = replica): a second plane of the same shape,
+// MOFF bytes above the lattice, all zero between rounds.  Each accepted
+// attempt ORs its replica's bit into its site's mark (returning the old
+// mark), then every attempt reads the marks of its five stencil sites, at the
+// lattice addresses plus MOFF.  An attempt whose stencil holds the site of an
+// accepted attempt of its replica from the same round sees that bit: at a
+// neighbour through the neighbour's mark; at its own site through the site's
+// mark if it did not accept, and if it did through the old mark returned to
+// one of the two ORs (a wave's lanes and its successive instructions reach
+// LDS in order).  Those attempts are the candidates; the exact loop sorts out
+// which accepted attempt came first.  Unlike the row and column masks, whose
+// unions admit a growing share of attempts as a group's accepted attempts
+// multiply, it admits none without an accepted attempt in the stencil.
+// The accepted attempts clear their bits at the end of the round.
+__device__ __forceinline__ unsigned lds_or_rtn32(unsigned addr, unsigned v) {
+    return __hip_atomic_fetch_or(lds_ptr32(addr), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_and32(unsigned addr, unsigned v) {
+    __hip_atomic_fetch_and(lds_ptr32(addr), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// The marks of a stencil (positions are byte offsets of the lattice dwords);
+// `own` replaces the centre's mark (the old mark of an accepted attempt).
+template <int MOFF>
+__device__ __forceinline__ unsigned stencil_marks(const Stencil &s, bool acc, unsigned own) {
+    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+    const unsigned c = lds_load32(M + (unsigned)s.c);
+    const unsigned n = lds_load32(M + (unsigned)s.up) | lds_load32(M + (unsigned)s.dn) |
+                       lds_load32(M + (unsigned)s.lt) | lds_load32(M + (unsigned)s.rt);
+    return n | (acc ? own : c);
+}
+
+// MOFF: 0 for the row/column conflict filter, else the byte offset of the
+// mark plane above the lattice (half words only).
+template <int LAT, int RNG, int S, bool POW2, int MOFF = 0>
 __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
+    static_assert(MOFF == 0 || (LAT == LAT_MULTIHALF && S >= 2 && S <= 32), "mark plane: half words, 2 to 32 slots");
     constexpr bool SPLIT = (LAT == LAT_MULTIHALF) || (LAT == LAT_MULTI16) || (LAT == LAT_MULTI8);
     constexpr bool MULTI = (LAT == LAT_MULTISPIN) || SPLIT;
     // replicas per workgroup in the multispin layouts (whole, half, quarter, eighth words)
@@ -557,6 +593,10 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
         const uint4 *src = (const uint4 *)((const unsigned *)a.lattice + (size_t)blockIdx.x * a.stride);
         uint4 *dst = (uint4 *)lat_lds;
         for (int i = tid; i < a.stride / 4; i += blockDim.x) dst[i] = src[i];
+        if constexpr (MOFF > 0) {
+            uint4 *mk = (uint4 *)(lat_lds + MOFF);
+            for (int i = tid; i < a.stride / 4; i += blockDim.x) mk[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
     } else if constexpr (LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
         // bits o .. o + RPG - 1 of plane (64 g + o) / 32 of each site's dword,
         // where replica r0 = blockIdx.x RPG is bit o of that plane
@@ -668,7 +708,23 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             // The filter pays up to 32 lanes per group (a 32-slot split launch
             // at L=64, 16384 replicas: 42.5 -> 37.9 ms); over a whole wave
             // (method 0 at 64 slots) it cost more than it saved (39.1 -> 40.6).
-            if constexpr (S <= 32) {
+            if constexpr (MOFF > 0) {
+                // the mark plane (see stencil_marks)
+                if (m != 0ull) {
+                    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+                    const unsigned bit = 1u << rl;
+                    const bool accA = __builtin_amdgcn_inverse_ballot_w64(mA);
+                    const bool accB = __builtin_amdgcn_inverse_ballot_w64(mB);
+                    unsigned ownA = 0u, ownB = 0u;
+                    if (accA) ownA = lds_or_rtn32(M + (unsigned)A.st.c, bit);
+                    if (accB) ownB = lds_or_rtn32(M + (unsigned)B.st.c, bit);
+                    asm volatile("" ::: "memory");
+                    const bool hA = (stencil_marks<MOFF>(A.st, accA, ownA) & bit) != 0u;
+                    const bool hB = (stencil_marks<MOFF>(B.st, accB, ownB) & bit) != 0u;
+                    const uint64_t cand = (eA & wave_mask(hA)) | (eB & wave_mask(hB));
+                    m = cand ? (m & group_span<S>(cand)) : 0ull;
+                }
+            } else if constexpr (S <= 32) {
                 // Filter (see nbr_mask): union the row and column masks of
                 // the accepted attempts of the earlier lanes of the group; an
                 // attempt within 1 of that union on both is a candidate.
@@ -716,6 +772,13 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             const uint64_t invB = invA | cB;
             if (__builtin_amdgcn_inverse_ballot_w64(mA & ~invA)) lat.flip(A.st.c, A.s01);
             if (__builtin_amdgcn_inverse_ballot_w64(mB & ~invB)) lat.flip(B.st.c, B.s01);
+            if constexpr (MOFF > 0) {
+                // clear the round's marks (every accepted attempt set one)
+                constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+                asm volatile("" ::: "memory");
+                if (__builtin_amdgcn_inverse_ballot_w64(mA)) lds_and32(M + (unsigned)A.st.c, ~(1u << rl));
+                if (__builtin_amdgcn_inverse_ballot_w64(mB)) lds_and32(M + (unsigned)B.st.c, ~(1u << rl));
+            }
             // The valid attempts of a group are a prefix of its order A, B,
             // A, B, ... (an attempt is invalid once an earlier one conflicts);
             // the next round starts after that prefix.  Disabled attempts

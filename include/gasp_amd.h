@@ -226,6 +226,15 @@ void gasp_set_tuning(int slots_per_replica, int waves_per_group);
 void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
                      int *waves_per_group);
 
+/* Conflict filter of method 2 with 32-bit half words and 2 to 32 slots, for
+ * engines created afterwards: 1 = row and column masks scanned over the
+ * lanes of a replica, 2 = a mark plane beside the lattice in LDS (L <= 90;
+ * larger lattices keep the masks), 0 (or any other value) = automatic: the
+ * mark plane where its LDS leaves as many workgroups resident as the launch
+ * fills.  Trajectories do not depend on it.  No counterpart in the reference
+ * (a tuning knob of this engine, like gasp_set_tuning). */
+void gasp_set_conflict_filter(int filter);
+
 /* Statistics of the most recent mc_driver_gpu / gasp_engine run on this
  * thread: sweep-kernel time (HIP events), attempts performed per replica,
  * kernel launches. */
@@ -332,6 +341,9 @@ void *gasp_engine_stream(gasp_engine *e);
 /* The engine's launch shape: slots per replica and waves per workgroup (the
  * word split for method 2), as gasp_set_tuning takes them. */
 int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves);
+/* The engine's conflict filter, as gasp_set_conflict_filter takes it (1 for
+ * engines without the choice). */
+int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter);
 
 /* ------------------------------------------------ batched graph, autotuner */
 /* One captured hipGraph advancing n engines concurrently (uneven L, methods,

@@ -102,6 +102,86 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
     assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
 
 
+@pytest.mark.parametrize("filt", [1, 2])
+@pytest.mark.parametrize("S", [2, 4, 16, 32])
+def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, S):
+    """Method 2 on half words with either conflict filter (1 row/column
+    masks, 2 the mark plane): both streams; random starts at beta = 0.3,
+    where half the attempts accept and conflicts are common, and the uniform
+    start at the reference's (beta, h); L = 17 (N odd, so the second launch
+    starts at an odd attempt and its first round with a disabled attempt);
+    L = 64 with a partial last word group; L = 72, whose mark plane sits
+    32 KB above the lattice instead of 16 KB."""
+    G, O = engine_lib, oracle
+    G.set_conflict_filter(filt)
+
+    def run(L, R, g0, seed, mt, beta, chunks):
+        G.set_tuning(S, 2)
+        try:
+            rng = G.RngState(R, seed, mt=mt)
+            with G.Engine(L, R, 2, rng, beta, H) as e:
+                assert e.conflict_filter() == filt
+                e.set_grids(g0)
+                for n in chunks:
+                    e.sweep(n)
+                out, mag = e.get_grids(), e.magnetisation()
+            rng.free()
+        finally:
+            G.set_tuning(0, 0)
+        return out, mag
+
+    try:
+        for L in (16, 17):
+            R, seed = 96, 1234567
+            print(f"[filter mt] filter={filt} S={S} L={L}", flush=True)
+            g0 = _start("random", R, L, seed)
+            got, _ = run(L, R, g0, seed, True, 0.3, [3, 2])
+            assert np.array_equal(got, O.sweep_mt_per_replica(L, g0.copy(), seed, 5, 0.3, H))
+        for L, R, start, beta, chunks in ((16, 100, "random", 0.3, [4, 3]), (16, 100, "down", BETA, [4, 3]),
+                                          (17, 100, "random", 0.3, [3, 2]), (64, 100, "down", BETA, [6, 4]),
+                                          (72, 40, "random", 0.3, [1, 1])):
+            seed = 987654321
+            print(f"[filter philox] filter={filt} S={S} L={L} start={start}", flush=True)
+            g0 = _start(start, R, L, seed)
+            got, mag = run(L, R, g0, seed, False, beta, chunks)
+            ref = O.sweep_philox(L, g0.copy(), seed, 0, sum(chunks) * L * L, beta, H)
+            assert np.array_equal(got, ref)
+            assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
+    finally:
+        G.set_conflict_filter(0)
+
+
+def test_conflict_filter_choice(gpu, engine_lib):
+    """The mark plane needs 4 N <= 32768 (L <= 90) and is chosen by default
+    only where its doubled LDS leaves enough workgroups resident: at L = 64
+    for 16384 replicas (two workgroups per CU), not for 524288 replicas on
+    8 slots (sixty-four workgroups per CU to place).  Whole words, other
+    methods and one slot per replica have no choice."""
+    G = engine_lib
+
+    def filt(L, R, method=2, S=16, W=2):
+        G.set_tuning(S, W)
+        try:
+            rng = G.RngState(R, 5)
+            with G.Engine(L, R, method, rng, BETA, H) as e:
+                return e.conflict_filter()
+        finally:
+            G.set_tuning(0, 0)
+            rng.free()
+
+    try:
+        G.set_conflict_filter(2)
+        assert (filt(90, 64), filt(91, 64)) == (2, 1)
+        G.set_conflict_filter(0)
+        assert filt(64, 16384) == 2
+        assert filt(64, 524288, S=8) == 1
+        assert filt(64, 64, S=16, W=1) == 1
+        assert filt(64, 64, S=1) == 1
+        assert filt(64, 64, method=1, S=8, W=1) == 1
+    finally:
+        G.set_conflict_filter(0)
+
+
 @pytest.mark.parametrize("S", [128, 256, 512, 1024])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
     """A replica spread over 2, 4 or 8 waves at the batched run's L = 256
