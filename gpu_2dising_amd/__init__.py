@@ -30,7 +30,7 @@ EXPORTS = (
     "compute_magnetisation_cpu", "compute_largest_cluster_cpu", "mc_driver_cpu",
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats", "gasp_set_conflict_filter",
-    "gasp_engine_get_conflict_filter",
+    "gasp_engine_get_conflict_filter", "gasp_set_lattice_staging", "gasp_engine_get_lattice_staging",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
     "gasp_engine_gather_grids", "gasp_engine_get_gridstates",
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
@@ -130,6 +130,7 @@ def lib() -> C.CDLL:
     L.gasp_free.argtypes = [vp]
     L.gasp_set_tuning.argtypes = [C.c_int, C.c_int]
     L.gasp_set_conflict_filter.argtypes = [C.c_int]
+    L.gasp_set_lattice_staging.argtypes = [C.c_int]
     L.gasp_get_tuning.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip]
     L.gasp_get_run_stats.argtypes = [C.POINTER(RunStats)]
     L.gasp_engine_create.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double,
@@ -160,6 +161,7 @@ def lib() -> C.CDLL:
     L.gasp_engine_kernel_ms.restype = C.c_double
     L.gasp_engine_get_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.gasp_engine_get_conflict_filter.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gasp_engine_get_lattice_staging.argtypes = [vp, C.POINTER(C.c_int)]
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
     L.gasp_sweep_grids.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
@@ -321,6 +323,12 @@ class Engine:
         check(lib().gasp_engine_get_shape(self.ptr, C.byref(s), C.byref(w)))
         return s.value, w.value
 
+    def lattice_staging(self) -> int:
+        """The engine's lattice staging: 1 in HBM, 2 staged in LDS (method 0 over several waves)."""
+        f = C.c_int(0)
+        check(lib().gasp_engine_get_lattice_staging(self.ptr, C.byref(f)))
+        return f.value
+
     def conflict_filter(self) -> int:
         """The engine's conflict filter: 1 row/column masks, 2 mark plane."""
         f = C.c_int(0)
@@ -471,6 +479,11 @@ def bootstrap_libc(fates: np.ndarray, m: int, seed: int = 1, nthreads: int = 0) 
 
 def set_tuning(slots: int = 0, waves: int = 0) -> None:
     lib().gasp_set_tuning(slots, waves)
+
+
+def set_lattice_staging(staging: int = 0) -> None:
+    """Method 0 over several waves: 1 lattice in HBM, 2 staged in LDS (0 = automatic)."""
+    lib().gasp_set_lattice_staging(staging)
 
 
 def set_conflict_filter(filter: int = 0) -> None:

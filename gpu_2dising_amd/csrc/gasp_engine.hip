@@ -98,6 +98,7 @@ uint64_t g_thr_f32[10];
 int g_tune_slots = 0;
 int g_tune_waves = 0;
 int g_tune_filter = 0;  // method 2 on half words: 1 row/column masks, 2 mark plane (0: automatic)
+int g_tune_stage = 0;   // method 0 over several waves: 1 lattice in HBM, 2 staged in LDS (0: automatic)
 thread_local gasp_run_stats_t t_stats{};
 
 // Thresholds in kernel order j = 5*s01 + (number of up neighbours).  With
@@ -218,6 +219,7 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
 }
 
 extern "C" void gasp_set_conflict_filter(int filter) { g_tune_filter = filter; }
+extern "C" void gasp_set_lattice_staging(int staging) { g_tune_stage = staging; }
 
 static int pow2_ceil(long long x) {
     int p = 1;
@@ -243,15 +245,29 @@ static int layout_stride(int method, int N) {
 // share each entry, and a 20-sweep launch there takes 3.98 ms against 4.33
 // with 8192 entries; the batched replay, whose other groups then find less
 // LDS on those CUs, still went from 5.5-5.8 to 5.4-5.5 ms.
-static int hbm_table_entries(int N, int R) {
+// `stage`: the replica's bytes are staged in LDS below the tables (their
+// budget shrinks by the lattice; 0 entries when the lattice leaves no room).
+static int hbm_table_entries(int N, int R, bool stage) {
     DeviceInfo d = device_info();
     const int ncu = d.ok ? d.ncu : 256;
     const size_t lds = d.ok ? d.lds_block : 65536;
     const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
-    const size_t budget = (lds / (size_t)per_cu - (size_t)kLdsLattice) / 8;
+    const size_t fixed = (size_t)kLdsLattice + (stage ? (size_t)layout_stride(0, N) : 0);
+    const size_t per_wg = lds / (size_t)per_cu;
+    if (per_wg < fixed + 16) return 0;
+    const size_t budget = (per_wg - fixed) / 8;
     int M = 1;
     while (M < N && M < 16384 && (size_t)M * 2 <= budget) M *= 2;
     return M;
+}
+
+// Method 0 over several waves stages the replica's bytes in LDS (the rounds
+// then wait on LDS instead of L2) where they fit beside tables of at least
+// min(N, 4096) entries for the workgroups a CU should hold.
+static bool hbm_stage(int N, int R) {
+    if (g_tune_stage == 1) return false;
+    const int M = hbm_table_entries(N, R, true);
+    return M > 0 && (g_tune_stage == 2 || M >= std::min(N, 4096));
 }
 
 // Method 2: `waves` is the word split (1, 2, 4, 8: 64, 32, 16, 8 replicas per
@@ -263,7 +279,10 @@ static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
         const int rpw = std::max(1, waves * 64 / S);
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
     }
-    if (S > 64) return (size_t)kLdsLattice + 8 * (size_t)hbm_table_entries(N, R);  // sweep_hbm_waves tables
+    if (S > 64) {  // sweep_hbm_waves: the tables, after the staged lattice
+        const bool stage = hbm_stage(N, R);
+        return (size_t)kLdsLattice + (stage ? (size_t)stride : 0) + 8 * (size_t)hbm_table_entries(N, R, stage);
+    }
     return (size_t)kLdsLattice;
 }
 
@@ -420,13 +439,14 @@ static SweepFn pick_marks(int S) {
     }
 }
 
-// moff: 0 (row/column filter) or the mark plane's offset above the half-word
-// lattice (16384 or 32768 bytes).
+// variant: for the half-word layout 0 (row/column filter) or the mark
+// plane's offset above the lattice (16384 or 32768 bytes); for the byte
+// lattice over several waves 1 if it is staged in LDS.
 template <int LAT, int RNG, bool P2>
-static SweepFn pick_sweep(int S, int moff) {
+static SweepFn pick_sweep(int S, int variant) {
     if constexpr (LAT == LAT_MULTIHALF) {
-        if (moff == 16384) return pick_marks<LAT, RNG, P2, 16384>(S);
-        if (moff == 32768) return pick_marks<LAT, RNG, P2, 32768>(S);
+        if (variant == 16384) return pick_marks<LAT, RNG, P2, 16384>(S);
+        if (variant == 32768) return pick_marks<LAT, RNG, P2, 32768>(S);
     }
     switch (S) {
         case 1: return sweep_kernel<LAT, RNG, 1, P2>;
@@ -443,17 +463,24 @@ static SweepFn pick_sweep(int S, int moff) {
         if (S == 64) return sweep_kernel<LAT, RNG, 64, P2>;
     }
     if constexpr (LAT == LAT_HBM) {
-        if (S == 128) return sweep_hbm_waves<RNG, 2, P2>;
-        if (S == 256) return sweep_hbm_waves<RNG, 4, P2>;
-        if (S == 512) return sweep_hbm_waves<RNG, 8, P2>;
-        if (S == 1024) return sweep_hbm_waves<RNG, 16, P2>;
+        if (variant == 1) {
+            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, true>;
+            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, true>;
+            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, true>;
+            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, true>;
+        } else {
+            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, false>;
+            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, false>;
+            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, false>;
+            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, false>;
+        }
     }
     return nullptr;
 }
 
 template <int LAT, int RNG>
-static SweepFn pick_sweep2(int S, bool pow2, int moff = 0) {
-    return pow2 ? pick_sweep<LAT, RNG, true>(S, moff) : pick_sweep<LAT, RNG, false>(S, moff);
+static SweepFn pick_sweep2(int S, bool pow2, int variant = 0) {
+    return pow2 ? pick_sweep<LAT, RNG, true>(S, variant) : pick_sweep<LAT, RNG, false>(S, variant);
 }
 
 template <int WB, bool P2>
@@ -473,11 +500,13 @@ static SweepFn sliced_fn(int waves, int split, bool pow2) {
     return pow2 ? pick_sliced<64, true>(waves) : pick_sliced<64, false>(waves);
 }
 
-static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int moff = 0) {
+// variant: see pick_sweep (method 2 on half words: the mark plane's offset;
+// method 0 over several waves: 1 = lattice staged in LDS).
+static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int variant = 0) {
     if (method == 3) return rng == 0 ? sliced_fn(S, W, pow2) : nullptr;
     if (method == 2 && W == 2)
-        return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2, moff)
-                        : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2, moff);
+        return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2, variant)
+                        : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2, variant);
     if (method == 2 && W == 4)
         return rng == 0 ? pick_sweep2<LAT_MULTI16, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI16, RNG_DRAWS>(S, pow2);
     if (method == 2 && W == 8)
@@ -486,7 +515,8 @@ static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int moff =
         return rng == 0 ? pick_sweep2<LAT_MULTISPIN, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTISPIN, RNG_DRAWS>(S, pow2);
     if (method == 1)
         return rng == 0 ? pick_sweep2<LAT_BITS, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_BITS, RNG_DRAWS>(S, pow2);
-    return rng == 0 ? pick_sweep2<LAT_HBM, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_HBM, RNG_DRAWS>(S, pow2);
+    return rng == 0 ? pick_sweep2<LAT_HBM, RNG_PHILOX>(S, pow2, variant)
+                    : pick_sweep2<LAT_HBM, RNG_DRAWS>(S, pow2, variant);
 }
 
 static bool is_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
@@ -502,7 +532,8 @@ static int ilog2(int L) {
 // kernel for (method, L, R).
 struct LaunchShape {
     int S = 1, W = 1, stride = 0, rpw = 0, nblocks = 0, block = 0;
-    int tbl = 1;  // sweep_hbm_waves: conflict-table entries
+    int tbl = 1;         // sweep_hbm_waves: conflict-table entries
+    bool stage = false;  // sweep_hbm_waves: the lattice staged in LDS
     int ncu = 256;
     size_t lds = 0, lat_bytes = 0;
     bool sliced = false;  // method 3: sweep_sliced, S = waves per workgroup, W = word split (1: 64, 2: 32)
@@ -534,7 +565,8 @@ static int launch_shape(int method, int L, int R, LaunchShape *sh) {
     DeviceInfo di = device_info();
     if (di.ok) sh->ncu = di.ncu;
     sh->lds = lds_bytes_for(method, N, sh->S, sh->W, R);
-    sh->tbl = (method == 0 && sh->S > 64) ? hbm_table_entries(N, R) : 1;
+    sh->stage = method == 0 && sh->S > 64 && hbm_stage(N, R);
+    sh->tbl = (method == 0 && sh->S > 64) ? hbm_table_entries(N, R, sh->stage) : 1;
     if (method == 2) {
         const int groups = (R + 63) / 64;
         sh->rpw = 64 / sh->W;
@@ -603,6 +635,7 @@ struct gasp_engine {
     int freeze = 0;          // skip replicas whose fate is decided
     int S0 = 1;              // slots per replica chosen at creation
     int moff = 0;            // mark plane offset above the lattice (0: row/column filter)
+    bool stage = false;      // method 0 over several waves: the lattice staged in LDS
     bool adapt = false;      // raise the slots as replicas are decided (see engine_adapt_slots)
     int decided = 0;         // decided replicas at the last counts read (or set_fates)
     int ncu = 256;
@@ -697,11 +730,12 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->moff = choose_filter(method, sh);
     if (e->moff) e->lds = marks_lds_bytes(e->moff, e->stride);
     e->tbl = sh.tbl;
+    e->stage = sh.stage;
     e->rpw = sh.rpw;
     e->block = sh.block;
     e->nblocks = sh.nblocks;
     e->lat_bytes = sh.lat_bytes;
-    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L), e->moff);
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L), e->stage ? 1 : e->moff);
     if (!fn) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
@@ -817,7 +851,8 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.tbl_mask = (unsigned)(e->tbl - 1);
     a.dn_thr = dn;
     a.up_thr = up;
-    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L), e->moff);
+    SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L),
+                          e->stage ? 1 : e->moff);
     if (check != CHECK_NONE) {
         hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
         HIP_TRY(hipGetLastError());
@@ -1207,6 +1242,12 @@ extern "C" int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter) {
     return GASP_OK;
 }
 
+extern "C" int gasp_engine_get_lattice_staging(gasp_engine *e, int *staging) {
+    if (!e || !staging) return set_error(GASP_ERR_ARG, "gasp_engine_get_lattice_staging: NULL argument");
+    *staging = e->stage ? 2 : 1;
+    return GASP_OK;
+}
+
 // ============================================================ caller grids
 // Stream-ordered sweeps of caller-owned int32 device grids (the layout
 // gpuInitGrid makes): the granularity of the reference driver's per-interval
@@ -1269,7 +1310,7 @@ static int sweep_grids_run(int L, int R, int method, gasp_rng_state *d_state, do
     else
         hipLaunchKernelGGL(pack_hbm, grid, blk, 0, st, d_grids, (unsigned char *)lat, R, N, sh.stride, 1);
     HIP_TRY(hipGetLastError());
-    SweepFn fn = sweep_fn(method, 0, sh.S, sh.W, is_pow2(L));
+    SweepFn fn = sweep_fn(method, 0, sh.S, sh.W, is_pow2(L), sh.stage ? 1 : 0);
     if (!fn) return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", sh.S);
     if (sh.lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
@@ -1664,6 +1705,11 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
                 cand.push_back({S, 8});
             }
         } else {
+            // method 0 with N >= 4096: fewer than 16 lanes per replica read every
+            // replica's bytes through L2 at once (at L = 256 with 256 replicas
+            // the probes of 1, 2 and 4 lanes took 260, 125 and 30 ms against
+            // 2-4 ms for the multi-wave shapes); not probed
+            if (method == 0 && L * L >= 4096 && S < 16) continue;
             for (int W = 1; W <= 4; W <<= 1) cand.push_back({S, W});
         }
     }
@@ -1672,9 +1718,10 @@ extern "C" int gasp_autotune(int method, int L, int ngrids, double beta, double 
         cand.push_back({256, 4});
         cand.push_back({512, 8});
         // not 1024 (sixteen waves): alone, the batched L = 256 group's 20 sweeps
-        // take 3.41 ms against 3.97 on eight waves, but in the batched graph
-        // its 1024-lane workgroups crowd the other groups' (7.0-7.2 ms per
-        // replay against 5.35-5.38), and the tuner probes a group alone
+        // take 2.60 ms against 2.63 on eight waves (lattice staged in LDS; 3.41
+        // against 3.97 from HBM), and in the batched graph its 1024-lane
+        // workgroups crowd the other groups' (7.0-7.2 ms per replay against
+        // 5.35-5.38 from HBM), while the tuner probes a group alone
     }
     gasp_rng_state *rng = nullptr;
     if (rng_alloc(ngrids, 12345, KIND_PHILOX, &rng)) return gasp_last_error();

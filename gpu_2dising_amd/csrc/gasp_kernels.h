@@ -35,7 +35,7 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 constexpr uint32_t kPhiloxDomain = 0x47415350u;  // "GASP"
 
-enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3, LAT_MULTI16 = 4, LAT_MULTI8 = 5 };
+enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3, LAT_MULTI16 = 4, LAT_MULTI8 = 5, LAT_LDS8 = 6 };
 enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
 enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
@@ -237,6 +237,9 @@ struct Geometry {
 //   no 64-bit address arithmetic per read, and reads past the workgroup's
 //   replicas return 0.  The sweeps order a round's flips before the next
 //   round's reads with fences (one wave) or barriers (several).
+// Layout 6 (layout 0 staged in LDS): one replica's bytes at LDS address
+//   kLdsLattice + site, for the multi-wave sweep of one replica per workgroup;
+//   its barriers order the flips (plain byte stores) before the next reads.
 template <int LAT>
 struct Lattice {
     unsigned char *base;  // LDS (1/2) or global (0) base of this lane's replica
@@ -258,6 +261,8 @@ struct Lattice {
                                               (uint32_t)rl, 1u);
         } else if constexpr (LAT == LAT_BITS) {
             return (base[site >> 3] >> (site & 7)) & 1;
+        } else if constexpr (LAT == LAT_LDS8) {
+            return (int)*(const lds_u8 *)(uintptr_t)(kLdsLattice + (unsigned)site);
         } else {
             return (int)__builtin_amdgcn_raw_buffer_load_b8(rsrc, off + (unsigned)site, 0, 0);
         }
@@ -272,6 +277,8 @@ struct Lattice {
             lds_xor32(kLdsLattice + ((unsigned)site & ~3u), 1u << ((((unsigned)site & 3u) << 3) + (unsigned)rl));
         } else if constexpr (LAT == LAT_BITS) {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
+        } else if constexpr (LAT == LAT_LDS8) {
+            *(lds_u8 *)(uintptr_t)(kLdsLattice + (unsigned)site) = (unsigned char)(old ^ 1);
         } else {
             __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(old ^ 1), rsrc, off + (unsigned)site, 0, 0);
         }
@@ -923,16 +930,21 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 // table during the next round.  Two barriers per round: after the marks,
 // and after the flips (which the next round's reads must see) and the counts
 // of deferred attempts per wave.
-template <int RNG, int W, bool POW2>
+// STAGE: the replica's bytes are staged in LDS (layout 6) at kLdsLattice for
+// the launch, and the tables follow them; otherwise they stay in HBM.
+template <int RNG, int W, bool POW2, bool STAGE>
 __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     static_assert(W == 2 || W == 4 || W == 8 || W == 16, "2, 4, 8 or 16 waves per replica");
     constexpr int S = 64 * W;
+    constexpr int LAT = STAGE ? LAT_LDS8 : LAT_HBM;
     constexpr unsigned kNone = 0xffffffffu;
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
     unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);  // [0]: up spins; [16 + 16 par + wave]: deferred attempts
     const unsigned hm = a.tbl_mask;
-    const unsigned tsz = 4u * (hm + 1u);  // bytes per table; table p at kLdsLattice + p tsz
+    const unsigned tsz = 4u * (hm + 1u);  // bytes per table; table p at tbase + p tsz
+    // a.stride (bytes per replica) is a multiple of 16
+    const unsigned tbase = (unsigned)kLdsLattice + (STAGE ? (unsigned)a.stride : 0u);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -947,15 +959,19 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
 
     stage_thresholds(a, thr_lds, tid);
     if (tid == 0) cnt_lds[0] = 0u;
-    for (unsigned i = (unsigned)tid; i < 2u * (hm + 1u); i += S) *lds_ptr32(kLdsLattice + 4u * i) = kNone;
-    __syncthreads();
-
-    Lattice<LAT_HBM> lat;
+    for (unsigned i = (unsigned)tid; i < 2u * (hm + 1u); i += S) *lds_ptr32(tbase + 4u * i) = kNone;
+    Lattice<LAT> lat;
     lat.base = (unsigned char *)a.lattice + (size_t)r * a.stride;
     lat.dw = nullptr;
     lat.rl = 0;
     lat.rsrc = __builtin_amdgcn_make_buffer_rsrc(lat.base, (short)0, a.stride, kBufferWord3);
     lat.off = 0u;
+    if constexpr (STAGE) {
+        const uint4 *src = (const uint4 *)lat.base;
+        uint4 *dst = (uint4 *)(smem + kLdsLattice);
+        for (int i = tid; i < a.stride / 16; i += S) dst[i] = src[i];
+    }
+    __syncthreads();
 
     // attempt l of the launch is global attempt a_begin + l: Philox pair
     // (a_begin + l) >> 1, first or second half by its low bit
@@ -991,19 +1007,19 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
                 st = geo.mt(d[0], d[1]);
                 ua = d[2];
             }
-            __hip_atomic_fetch_min(lds_ptr32(kLdsLattice + par * tsz + 4u * ((unsigned)st.c & hm)), l,
+            __hip_atomic_fetch_min(lds_ptr32(tbase + par * tsz + 4u * ((unsigned)st.c & hm)), l,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const Spins x = read_spins<LAT_HBM>(lat, st);
+            const Spins x = read_spins<LAT>(lat, st);
             const Attempt t = decide(st, x, ua);
             acc = t.acc;
             s01 = t.s01;
         }
         // the previous round's mark, in the other table (read by nobody now)
-        if (had_prev) *lds_ptr32(kLdsLattice + (par ^ 1u) * tsz + 4u * prev_ent) = kNone;
+        if (had_prev) *lds_ptr32(tbase + (par ^ 1u) * tsz + 4u * prev_ent) = kNone;
         __syncthreads();
         bool deferred = false;
         if (has) {
-            const unsigned tb = kLdsLattice + par * tsz;
+            const unsigned tb = tbase + par * tsz;
             const uint32_t m = min(min(min(lds_load32(tb + 4u * ((unsigned)st.c & hm)),
                                            lds_load32(tb + 4u * ((unsigned)st.up & hm))),
                                        min(lds_load32(tb + 4u * ((unsigned)st.dn & hm)),
@@ -1044,9 +1060,20 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     // ---------------------------------------------------------- epilogue
     __syncthreads();
     unsigned c = 0;
-    const unsigned *w = (const unsigned *)lat.base;
-    for (int i = tid; i < a.stride / 4; i += S)
-        c += __popc(__hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0x01010101u);
+    if constexpr (STAGE) {
+        const uint4 *src = (const uint4 *)(smem + kLdsLattice);
+        uint4 *dst = (uint4 *)lat.base;
+        for (int i = tid; i < a.stride / 16; i += S) {
+            const uint4 v = src[i];
+            c += __popc(v.x & 0x01010101u) + __popc(v.y & 0x01010101u) + __popc(v.z & 0x01010101u) +
+                 __popc(v.w & 0x01010101u);
+            dst[i] = v;
+        }
+    } else {
+        const unsigned *w = (const unsigned *)lat.base;
+        for (int i = tid; i < a.stride / 4; i += S)
+            c += __popc(__hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0x01010101u);
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
     if (lane == 0) atomicAdd(&cnt_lds[0], c);

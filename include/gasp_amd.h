@@ -235,6 +235,15 @@ void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
  * (a tuning knob of this engine, like gasp_set_tuning). */
 void gasp_set_conflict_filter(int filter);
 
+/* Method 0 with 128 to 1024 slots (one replica over 2 to 16 waves), for
+ * engines created afterwards: 1 = the byte lattice stays in HBM (read through
+ * L2 every round), 2 = it is staged in LDS for each launch where it fits beside
+ * the dependence tables (L up to about 330 at one workgroup per CU), 0 (or any
+ * other value) = automatic: staged where the tables keep min(N, 4096) entries.
+ * Trajectories do not depend on it.  No counterpart in the reference (a tuning
+ * knob of this engine, like gasp_set_tuning). */
+void gasp_set_lattice_staging(int staging);
+
 /* Statistics of the most recent mc_driver_gpu / gasp_engine run on this
  * thread: sweep-kernel time (HIP events), attempts performed per replica,
  * kernel launches. */
@@ -344,6 +353,9 @@ int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves);
 /* The engine's conflict filter, as gasp_set_conflict_filter takes it (1 for
  * engines without the choice). */
 int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter);
+/* The engine's lattice staging, as gasp_set_lattice_staging takes it (1 for
+ * engines without the choice). */
+int gasp_engine_get_lattice_staging(gasp_engine *e, int *staging);
 
 /* ------------------------------------------------ batched graph, autotuner */
 /* One captured hipGraph advancing n engines concurrently (uneven L, methods,

@@ -133,8 +133,8 @@ def test_graph_after_direct_sweeps(gpu, engine_lib, oracle):
 
 def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
     """The batched configuration at its real shapes: L = 32, 48, 64, 96 (LDS
-    multispin) with 8192, 4096, 4096, 2048 replicas and L = 256 (HBM path) with
-    256, beta and h drawn per group, itask 0 and 1 alternating, launch shapes and
+    multispin) with 8192, 4096, 4096, 2048 replicas and L = 256 (method 0, its
+    byte lattice staged in LDS) with 256, beta and h drawn per group, itask 0 and 1 alternating, launch shapes and
     sweeps per launch from the autotuner, three replays of 20 sweeps.  For 12
     replicas of every group (first, last and random ones) the oracle replays
     the same Philox streams: grids, magnetisations and fates must be equal; the
@@ -144,6 +144,9 @@ def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
     G = engine_lib
     print("[batched] five groups at full shape, autotuned", flush=True)
     b = Batch(chk=20, seed=2026)
+    # the groups keep the row/column filter; the L = 256 lattice is staged in LDS
+    assert all(g.engine.conflict_filter() == 1 for g in b.groups)
+    assert [g.engine.lattice_staging() for g in b.groups if g.method == 0] == [2]
     replays = 3
     b.graph.launch(replays)
     rs = np.random.default_rng(5)

@@ -182,21 +182,33 @@ def test_conflict_filter_choice(gpu, engine_lib):
         G.set_conflict_filter(0)
 
 
+@pytest.mark.parametrize("staging", [1, 2])
 @pytest.mark.parametrize("S", [128, 256, 512, 1024])
-def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S):
-    """A replica spread over 2, 4 or 8 waves at the batched run's L = 256
+def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S, staging):
+    """A replica spread over 2, 4, 8 or 16 waves at the batched run's L = 256
     shape (R = 6, so the conflict table has fewer entries than sites and
-    sites share entries), from all -1 at the reference's (beta, h) and from
-    a random start at beta = 0.3 (about half the attempts accepted)."""
+    sites share entries), with the byte lattice in HBM (staging 1) or staged
+    in LDS (2), from all -1 at the reference's (beta, h) and from a random
+    start at beta = 0.3 (about half the attempts accepted)."""
     G, O = engine_lib, oracle
     L, R, seed = 256, 6, 4242
-    for start, beta in (("down", BETA), ("random", 0.3)):
-        print(f"[L256 waves] S={S} start={start}", flush=True)
-        g0 = _start(start, R, L, seed)
-        got, mag = _run_engine(G, L, R, 0, S, g0, seed, False, beta, H, [1, 2], W=S // 64)
-        ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
-        assert np.array_equal(got, ref)
-        assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
+    G.set_lattice_staging(staging)
+    try:
+        rng = G.RngState(R, 1)
+        G.set_tuning(S, S // 64)
+        with G.Engine(L, R, 0, rng, BETA, H) as e:
+            assert e.lattice_staging() == staging
+        G.set_tuning(0, 0)
+        rng.free()
+        for start, beta in (("down", BETA), ("random", 0.3)):
+            print(f"[L256 waves] S={S} staging={staging} start={start}", flush=True)
+            g0 = _start(start, R, L, seed)
+            got, mag = _run_engine(G, L, R, 0, S, g0, seed, False, beta, H, [1, 2], W=S // 64)
+            ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
+            assert np.array_equal(got, ref)
+            assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
+    finally:
+        G.set_lattice_staging(0)
 
 
 def test_hbm_waves_small_table(gpu, engine_lib, oracle):
