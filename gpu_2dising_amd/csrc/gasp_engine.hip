@@ -253,8 +253,8 @@ static int hbm_table_entries(int N, int R, bool stage) {
     const size_t lds = d.ok ? d.lds_block : 65536;
     const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
     const size_t fixed = (size_t)kLdsLattice + (stage ? (size_t)layout_stride(0, N) : 0);
-    const size_t per_wg = lds / (size_t)per_cu;
-    if (per_wg < fixed + 16) return 0;
+    const size_t per_wg = lds / (size_t)per_cu;  // at least 16 KB (at most four per CU)
+    if (per_wg < fixed + 16) return 0;            // only a staged lattice can leave no room
     const size_t budget = (per_wg - fixed) / 8;
     int M = 1;
     while (M < N && M < 16384 && (size_t)M * 2 <= budget) M *= 2;
