@@ -427,26 +427,41 @@ extern "C" void gasp_get_run_stats(gasp_run_stats_t *out) {
 // ============================================================ kernel table
 using SweepFn = void (*)(SweepArgs);
 
+// Mark-plane kernels: half words with 2 to 32 slots and both streams;
+// quarter and eighth words (few replicas) with 8 to 32 slots, Philox only.
 template <int LAT, int RNG, bool P2, int MOFF>
 static SweepFn pick_marks(int S) {
-    switch (S) {
-        case 2: return sweep_kernel<LAT, RNG, 2, P2, MOFF>;
-        case 4: return sweep_kernel<LAT, RNG, 4, P2, MOFF>;
-        case 8: return sweep_kernel<LAT, RNG, 8, P2, MOFF>;
-        case 16: return sweep_kernel<LAT, RNG, 16, P2, MOFF>;
-        case 32: return sweep_kernel<LAT, RNG, 32, P2, MOFF>;
-        default: return nullptr;
+    if constexpr (LAT == LAT_MULTIHALF) {
+        switch (S) {
+            case 2: return sweep_kernel<LAT, RNG, 2, P2, MOFF>;
+            case 4: return sweep_kernel<LAT, RNG, 4, P2, MOFF>;
+            default: break;
+        }
     }
+    if constexpr (LAT == LAT_MULTIHALF || RNG == RNG_PHILOX) {
+        switch (S) {
+            case 8: return sweep_kernel<LAT, RNG, 8, P2, MOFF>;
+            case 16: return sweep_kernel<LAT, RNG, 16, P2, MOFF>;
+            case 32: return sweep_kernel<LAT, RNG, 32, P2, MOFF>;
+            default: break;
+        }
+    }
+    return nullptr;
 }
 
-// variant: for the half-word layout 0 (row/column filter) or the mark
-// plane's offset above the lattice (16384 or 32768 bytes); for the byte
+// variant: for the split-word layouts 0 (row/column filter) or the mark
+// plane's offset above the lattice (4096 .. 32768 bytes); for the byte
 // lattice over several waves 1 if it is staged in LDS.
 template <int LAT, int RNG, bool P2>
 static SweepFn pick_sweep(int S, int variant) {
-    if constexpr (LAT == LAT_MULTIHALF) {
-        if (variant == 16384) return pick_marks<LAT, RNG, P2, 16384>(S);
-        if (variant == 32768) return pick_marks<LAT, RNG, P2, 32768>(S);
+    if constexpr (LAT == LAT_MULTIHALF || LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
+        switch (variant) {
+            case 4096: return pick_marks<LAT, RNG, P2, 4096>(S);
+            case 8192: return pick_marks<LAT, RNG, P2, 8192>(S);
+            case 16384: return pick_marks<LAT, RNG, P2, 16384>(S);
+            case 32768: return pick_marks<LAT, RNG, P2, 32768>(S);
+            default: break;
+        }
     }
     switch (S) {
         case 1: return sweep_kernel<LAT, RNG, 1, P2>;
@@ -500,7 +515,7 @@ static SweepFn sliced_fn(int waves, int split, bool pow2) {
     return pow2 ? pick_sliced<64, true>(waves) : pick_sliced<64, false>(waves);
 }
 
-// variant: see pick_sweep (method 2 on half words: the mark plane's offset;
+// variant: see pick_sweep (method 2 on split words: the mark plane's offset;
 // method 0 over several waves: 1 = lattice staged in LDS).
 static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int variant = 0) {
     if (method == 3) return rng == 0 ? sliced_fn(S, W, pow2) : nullptr;
@@ -508,9 +523,11 @@ static SweepFn sweep_fn(int method, int rng, int S, int W, bool pow2, int varian
         return rng == 0 ? pick_sweep2<LAT_MULTIHALF, RNG_PHILOX>(S, pow2, variant)
                         : pick_sweep2<LAT_MULTIHALF, RNG_DRAWS>(S, pow2, variant);
     if (method == 2 && W == 4)
-        return rng == 0 ? pick_sweep2<LAT_MULTI16, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI16, RNG_DRAWS>(S, pow2);
+        return rng == 0 ? pick_sweep2<LAT_MULTI16, RNG_PHILOX>(S, pow2, variant)
+                        : pick_sweep2<LAT_MULTI16, RNG_DRAWS>(S, pow2, variant);
     if (method == 2 && W == 8)
-        return rng == 0 ? pick_sweep2<LAT_MULTI8, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTI8, RNG_DRAWS>(S, pow2);
+        return rng == 0 ? pick_sweep2<LAT_MULTI8, RNG_PHILOX>(S, pow2, variant)
+                        : pick_sweep2<LAT_MULTI8, RNG_DRAWS>(S, pow2, variant);
     if (method == 2)
         return rng == 0 ? pick_sweep2<LAT_MULTISPIN, RNG_PHILOX>(S, pow2) : pick_sweep2<LAT_MULTISPIN, RNG_DRAWS>(S, pow2);
     if (method == 1)
@@ -587,23 +604,34 @@ static int launch_shape(int method, int L, int R, LaunchShape *sh) {
     return GASP_OK;
 }
 
-static size_t marks_lds_bytes(int moff, int stride) {
-    return (size_t)kLdsLattice + (size_t)moff + 4 * (size_t)stride;
+// LDS of a split-word workgroup (W = 2, 4, 8: 4, 2, 1 bytes per site) with the mark plane
+static size_t marks_lds_bytes(int moff, int W, int stride) {
+    return (size_t)kLdsLattice + (size_t)moff + (size_t)(8 / W) * (size_t)stride;
 }
 
-// Conflict filter of method 2 on half words (W = 2) with 2 to 32 slots: the
-// mark plane (returns its offset above the lattice) where a second
-// lattice-sized plane fits below the 64 KB reach of a ds instruction's
-// offset (4 N <= 32768, L <= 90) and its LDS leaves as many workgroups
-// resident per CU as the launch can use; else 0, the row/column masks.  At
-// L = 64 a 100-sweep launch of 16384 replicas (two workgroups per CU) takes
-// 17.85 ms with the marks against 21.28 ms, but one of 524288 replicas takes
-// 453 ms against 412: there the doubled LDS halves the resident workgroups.
-static int choose_filter(int method, const LaunchShape &sh) {
-    if (method != 2 || sh.W != 2 || sh.S < 2 || sh.S > 32 || g_tune_filter == 1) return 0;
-    if (4 * sh.stride > 32768) return 0;
-    const int moff = 4 * sh.stride <= 16384 ? 16384 : 32768;
+// Conflict filter of method 2 on split words with 2 to 32 slots (quarter and
+// eighth words: 8 to 32, Philox only): the mark plane (returns its offset
+// above the lattice, the first of 4, 8, 16 or 32 KB that holds a lattice-sized
+// plane, within the 64 KB reach of a ds instruction's offset: up to L = 90 on
+// half words) where its LDS leaves as many workgroups resident per CU as the
+// launch can use; else 0, the row/column masks.  At L = 64 a 100-sweep launch
+// of 16384 replicas (two half-word workgroups per CU) takes 17.85 ms with the
+// marks against 21.28 ms, but one of 524288 replicas takes 453 ms against
+// 412: there the doubled LDS halves the resident workgroups.
+static int choose_filter(int method, int L, const LaunchShape &sh, bool mt) {
+    if (method != 2 || sh.W == 1 || sh.S < 2 || sh.S > 32 || g_tune_filter == 1) return 0;
+    if (sh.W != 2 && (mt || sh.S < 8)) return 0;
+    const size_t plane = (size_t)(8 / sh.W) * (size_t)sh.stride;
+    int moff = 4096;
+    while ((size_t)moff < plane) moff *= 2;
+    if (moff > 32768) return 0;
     if (g_tune_filter == 2) return moff;
+    // quarter and eighth words: only where the row/column masks are loose (32
+    // slots, 64 attempts per group, or rows and columns folded for L > 64): 20
+    // sweeps of 2048 replicas at L = 96 on eighth words, 32 slots, take 2.85 ms
+    // with the marks against 3.88, but 4096 at L = 64 on quarter words, 16
+    // slots (one workgroup of four waves per CU), 2.36 against 2.10
+    if (sh.W != 2 && sh.S < 32 && L <= 64) return 0;
     DeviceInfo d = device_info();
     const size_t lds_cu = d.ok ? d.lds_block : 65536;
     const int waves = std::max(1, (sh.block + 63) / 64);
@@ -612,7 +640,7 @@ static int choose_filter(int method, const LaunchShape &sh) {
     };
     const long long need = (sh.nblocks + sh.ncu - 1) / std::max(sh.ncu, 1);
     const long long plain = std::min(need, resident(sh.lds));
-    const long long marks = std::min(need, resident(marks_lds_bytes(moff, sh.stride)));
+    const long long marks = std::min(need, resident(marks_lds_bytes(moff, sh.W, sh.stride)));
     return marks >= plain ? moff : 0;
 }
 
@@ -727,8 +755,8 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->ncu = sh.ncu;
     e->lds = sh.lds;
     // the mark plane (if any) lies above the lattice, in the same allocation
-    e->moff = choose_filter(method, sh);
-    if (e->moff) e->lds = marks_lds_bytes(e->moff, e->stride);
+    e->moff = choose_filter(method, L, sh, e->rng.kind == KIND_MT);
+    if (e->moff) e->lds = marks_lds_bytes(e->moff, e->W, e->stride);
     e->tbl = sh.tbl;
     e->stage = sh.stage;
     e->rpw = sh.rpw;

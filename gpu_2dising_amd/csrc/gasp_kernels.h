@@ -501,13 +501,13 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
 }
 
 // ------------------------------------------------------------ mark plane
-// An exact candidate test for the speculation groups of the split-word layout
-// (a dword per site, bit rl = replica): a second plane of the same shape,
-// MOFF bytes above the lattice, all zero between rounds.  Each accepted
-// attempt ORs its replica's bit into its site's mark (returning the old
-// mark), then every attempt reads the marks of its five stencil sites, at the
-// lattice addresses plus MOFF.  An attempt whose stencil holds the site of an
-// accepted attempt of its replica from the same round sees that bit: at a
+// An exact candidate test for the speculation groups of the split-word layouts
+// (an entry of 32, 16 or 8 bits per site, bit rl = replica): a second plane of
+// the same shape, MOFF bytes above the lattice, all zero between rounds.  Each
+// accepted attempt ORs its replica's bit into its site's mark (returning the
+// old mark), then every attempt reads the marks of its five stencil sites, at
+// the lattice addresses plus MOFF.  An attempt whose stencil holds the site of
+// an accepted attempt of its replica from the same round sees that bit: at a
 // neighbour through the neighbour's mark; at its own site through the site's
 // mark if it did not accept, and if it did through the old mark returned to
 // one of the two ORs (a wave's lanes and its successive instructions reach
@@ -522,23 +522,66 @@ __device__ __forceinline__ unsigned lds_or_rtn32(unsigned addr, unsigned v) {
 __device__ __forceinline__ void lds_and32(unsigned addr, unsigned v) {
     __hip_atomic_fetch_and(lds_ptr32(addr), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-// The marks of a stencil (positions are byte offsets of the lattice dwords);
-// `own` replaces the centre's mark (the old mark of an accepted attempt).
+// The entry at byte position pos of a split-word plane: half words fill a
+// dword; quarter and eighth words share one, at bits 8 (pos mod 4) + rl.
+template <int LAT>
+__device__ __forceinline__ unsigned entry_shift(unsigned pos, int rl) {
+    if constexpr (LAT == LAT_MULTIHALF) return (unsigned)rl;
+    else return ((pos & 3u) << 3) + (unsigned)rl;
+}
+template <int LAT>
+__device__ __forceinline__ unsigned entry_dword(unsigned pos) {
+    if constexpr (LAT == LAT_MULTIHALF) return pos;
+    else return pos & ~3u;
+}
+template <int LAT>
+__device__ __forceinline__ unsigned read_entry(unsigned addr) {
+    if constexpr (LAT == LAT_MULTIHALF) return lds_load32(addr);
+    else if constexpr (LAT == LAT_MULTI16) return *(const lds_u16 *)(uintptr_t)addr;
+    else return *(const lds_u8 *)(uintptr_t)addr;
+}
+// Marks the site; returns whether its old mark held the replica's bit.
+template <int LAT, int MOFF>
+__device__ __forceinline__ bool mark_site(int pos, int rl) {
+    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+    const unsigned sh = entry_shift<LAT>((unsigned)pos, rl);
+    return ((lds_or_rtn32(M + entry_dword<LAT>((unsigned)pos), 1u << sh) >> sh) & 1u) != 0u;
+}
+template <int LAT, int MOFF>
+__device__ __forceinline__ void unmark_site(int pos, int rl) {
+    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+    const unsigned sh = entry_shift<LAT>((unsigned)pos, rl);
+    lds_and32(M + entry_dword<LAT>((unsigned)pos), ~(1u << sh));
+}
+// Half words: the OR of the stencil's marks, `own` (the old mark of an
+// accepted attempt) in place of the centre's.
 template <int MOFF>
-__device__ __forceinline__ unsigned stencil_marks(const Stencil &s, bool acc, unsigned own) {
+__device__ __forceinline__ unsigned half_marks(const Stencil &s, bool acc, unsigned own) {
     constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
     const unsigned c = lds_load32(M + (unsigned)s.c);
     const unsigned n = lds_load32(M + (unsigned)s.up) | lds_load32(M + (unsigned)s.dn) |
                        lds_load32(M + (unsigned)s.lt) | lds_load32(M + (unsigned)s.rt);
     return n | (acc ? own : c);
 }
+// Quarter and eighth words: whether the stencil's marks hold the replica's
+// bit; `own` stands for the centre's mark of an accepted attempt (its old
+// mark).
+template <int LAT, int MOFF>
+__device__ __forceinline__ bool stencil_hit(const Stencil &s, int rl, bool acc, bool own) {
+    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+    const unsigned c = read_entry<LAT>(M + (unsigned)s.c);
+    const unsigned n = read_entry<LAT>(M + (unsigned)s.up) | read_entry<LAT>(M + (unsigned)s.dn) |
+                       read_entry<LAT>(M + (unsigned)s.lt) | read_entry<LAT>(M + (unsigned)s.rt);
+    return ((n >> rl) & 1u) != 0u || (acc ? own : ((c >> rl) & 1u) != 0u);
+}
 
 // MOFF: 0 for the row/column conflict filter, else the byte offset of the
-// mark plane above the lattice (half words only).
+// mark plane above the lattice (split words).
 template <int LAT, int RNG, int S, bool POW2, int MOFF = 0>
 __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     static_assert(S >= 1 && S <= 64 && (S & (S - 1)) == 0, "S must be a power of two <= 64");
-    static_assert(MOFF == 0 || (LAT == LAT_MULTIHALF && S >= 2 && S <= 32), "mark plane: half words, 2 to 32 slots");
+    static_assert(MOFF == 0 || ((LAT == LAT_MULTIHALF || LAT == LAT_MULTI16 || LAT == LAT_MULTI8) && S >= 2 && S <= 32),
+                  "mark plane: split words, 2 to 32 slots");
     constexpr bool SPLIT = (LAT == LAT_MULTIHALF) || (LAT == LAT_MULTI16) || (LAT == LAT_MULTI8);
     constexpr bool MULTI = (LAT == LAT_MULTISPIN) || SPLIT;
     // replicas per workgroup in the multispin layouts (whole, half, quarter, eighth words)
@@ -613,6 +656,12 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             const unsigned v = src[i] >> o;
             if constexpr (LAT == LAT_MULTI16) ((unsigned short *)lat_lds)[i] = (unsigned short)v;
             else lat_lds[i] = (unsigned char)v;
+        }
+        if constexpr (MOFF > 0) {
+            // the plane's 2 or 1 bytes per site, a.stride (a multiple of 4) sites
+            unsigned *mk = (unsigned *)(lat_lds + MOFF);
+            const int ndw = (LAT == LAT_MULTI16 ? 2 : 1) * a.stride / 4;
+            for (int i = tid; i < ndw; i += blockDim.x) mk[i] = 0u;
         }
     } else if constexpr (LAT == LAT_BITS) {
         const int nrep = min(a.rpw, a.R - r0);
@@ -716,18 +765,29 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             // at L=64, 16384 replicas: 42.5 -> 37.9 ms); over a whole wave
             // (method 0 at 64 slots) it cost more than it saved (39.1 -> 40.6).
             if constexpr (MOFF > 0) {
-                // the mark plane (see stencil_marks)
+                // the mark plane (see mark_site)
                 if (m != 0ull) {
-                    constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
-                    const unsigned bit = 1u << rl;
                     const bool accA = __builtin_amdgcn_inverse_ballot_w64(mA);
                     const bool accB = __builtin_amdgcn_inverse_ballot_w64(mB);
-                    unsigned ownA = 0u, ownB = 0u;
-                    if (accA) ownA = lds_or_rtn32(M + (unsigned)A.st.c, bit);
-                    if (accB) ownB = lds_or_rtn32(M + (unsigned)B.st.c, bit);
-                    asm volatile("" ::: "memory");
-                    const bool hA = (stencil_marks<MOFF>(A.st, accA, ownA) & bit) != 0u;
-                    const bool hB = (stencil_marks<MOFF>(B.st, accB, ownB) & bit) != 0u;
+                    bool hA, hB;
+                    if constexpr (LAT == LAT_MULTIHALF) {
+                        // whole dwords: one test of the replica's bit at the end
+                        constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
+                        const unsigned bit = 1u << rl;
+                        unsigned ownA = 0u, ownB = 0u;
+                        if (accA) ownA = lds_or_rtn32(M + (unsigned)A.st.c, bit);
+                        if (accB) ownB = lds_or_rtn32(M + (unsigned)B.st.c, bit);
+                        asm volatile("" ::: "memory");
+                        hA = (half_marks<MOFF>(A.st, accA, ownA) & bit) != 0u;
+                        hB = (half_marks<MOFF>(B.st, accB, ownB) & bit) != 0u;
+                    } else {
+                        bool ownA = false, ownB = false;
+                        if (accA) ownA = mark_site<LAT, MOFF>(A.st.c, rl);
+                        if (accB) ownB = mark_site<LAT, MOFF>(B.st.c, rl);
+                        asm volatile("" ::: "memory");
+                        hA = stencil_hit<LAT, MOFF>(A.st, rl, accA, ownA);
+                        hB = stencil_hit<LAT, MOFF>(B.st, rl, accB, ownB);
+                    }
                     const uint64_t cand = (eA & wave_mask(hA)) | (eB & wave_mask(hB));
                     m = cand ? (m & group_span<S>(cand)) : 0ull;
                 }
@@ -781,10 +841,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             if (__builtin_amdgcn_inverse_ballot_w64(mB & ~invB)) lat.flip(B.st.c, B.s01);
             if constexpr (MOFF > 0) {
                 // clear the round's marks (every accepted attempt set one)
-                constexpr unsigned M = (unsigned)(kLdsLattice + MOFF);
                 asm volatile("" ::: "memory");
-                if (__builtin_amdgcn_inverse_ballot_w64(mA)) lds_and32(M + (unsigned)A.st.c, ~(1u << rl));
-                if (__builtin_amdgcn_inverse_ballot_w64(mB)) lds_and32(M + (unsigned)B.st.c, ~(1u << rl));
+                if (__builtin_amdgcn_inverse_ballot_w64(mA)) unmark_site<LAT, MOFF>(A.st.c, rl);
+                if (__builtin_amdgcn_inverse_ballot_w64(mB)) unmark_site<LAT, MOFF>(B.st.c, rl);
             }
             // The valid attempts of a group are a prefix of its order A, B,
             // A, B, ... (an attempt is invalid once an earlier one conflicts);

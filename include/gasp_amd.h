@@ -226,13 +226,16 @@ void gasp_set_tuning(int slots_per_replica, int waves_per_group);
 void gasp_get_tuning(int method, int L, int ngrids, int *slots_per_replica,
                      int *waves_per_group);
 
-/* Conflict filter of method 2 with 32-bit half words and 2 to 32 slots, for
+/* Conflict filter of method 2 on split words (half words with 2 to 32 slots;
+ * quarter and eighth words with 8 to 32 slots and Philox streams), for
  * engines created afterwards: 1 = row and column masks scanned over the
- * lanes of a replica, 2 = a mark plane beside the lattice in LDS (L <= 90;
- * larger lattices keep the masks), 0 (or any other value) = automatic: the
- * mark plane where its LDS leaves as many workgroups resident as the launch
- * fills.  Trajectories do not depend on it.  No counterpart in the reference
- * (a tuning knob of this engine, like gasp_set_tuning). */
+ * lanes of a replica, 2 = a mark plane beside the lattice in LDS (up to L =
+ * 90, 128 and 181 for half, quarter and eighth words; larger lattices keep
+ * the masks), 0 (or any other value) = automatic: the mark plane where its
+ * LDS leaves as many workgroups resident as the launch fills (on quarter and
+ * eighth words only with 32 slots or L > 64).  Trajectories do not depend on
+ * it.  No counterpart in the reference (a tuning knob of this engine, like
+ * gasp_set_tuning). */
 void gasp_set_conflict_filter(int filter);
 
 /* Method 0 with 128 to 1024 slots (one replica over 2 to 16 waves), for

@@ -103,24 +103,27 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
 
 
 @pytest.mark.parametrize("filt", [1, 2])
-@pytest.mark.parametrize("S", [2, 4, 16, 32])
-def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, S):
-    """Method 2 on half words with either conflict filter (1 row/column
-    masks, 2 the mark plane): both streams; random starts at beta = 0.3,
-    where half the attempts accept and conflicts are common, and the uniform
-    start at the reference's (beta, h); L = 17 (N odd, so the second launch
-    starts at an odd attempt and its first round with a disabled attempt);
-    L = 64 with a partial last word group; L = 72, whose mark plane sits
-    32 KB above the lattice instead of 16 KB."""
+@pytest.mark.parametrize("W,S", [(2, 2), (2, 4), (2, 16), (2, 32), (4, 8), (4, 16), (8, 8), (8, 32)])
+def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, W, S):
+    """Method 2 on split words (W = 2, 4, 8: half, quarter, eighth words)
+    with either conflict filter (1 row/column masks, 2 the mark plane; on
+    quarter and eighth words for Philox streams only): random starts at
+    beta = 0.3, where half the attempts accept and conflicts are common, and
+    the uniform start at the reference's (beta, h); L = 17 (N odd, so the
+    second launch starts at an odd attempt and its first round with a
+    disabled attempt); L = 64 with a partial last word group; L = 72, whose
+    mark plane sits 8 to 32 KB above the lattice by the layout."""
     G, O = engine_lib, oracle
     G.set_conflict_filter(filt)
+    marks = filt == 2
 
     def run(L, R, g0, seed, mt, beta, chunks):
-        G.set_tuning(S, 2)
+        G.set_tuning(S, W)
         try:
             rng = G.RngState(R, seed, mt=mt)
             with G.Engine(L, R, 2, rng, beta, H) as e:
-                assert e.conflict_filter() == filt
+                assert e.shape() == (S, W)
+                assert e.conflict_filter() == (2 if marks and (W == 2 or not mt) else 1)
                 e.set_grids(g0)
                 for n in chunks:
                     e.sweep(n)
@@ -133,7 +136,7 @@ def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, S):
     try:
         for L in (16, 17):
             R, seed = 96, 1234567
-            print(f"[filter mt] filter={filt} S={S} L={L}", flush=True)
+            print(f"[filter mt] filter={filt} W={W} S={S} L={L}", flush=True)
             g0 = _start("random", R, L, seed)
             got, _ = run(L, R, g0, seed, True, 0.3, [3, 2])
             assert np.array_equal(got, O.sweep_mt_per_replica(L, g0.copy(), seed, 5, 0.3, H))
@@ -141,7 +144,7 @@ def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, S):
                                           (17, 100, "random", 0.3, [3, 2]), (64, 100, "down", BETA, [6, 4]),
                                           (72, 40, "random", 0.3, [1, 1])):
             seed = 987654321
-            print(f"[filter philox] filter={filt} S={S} L={L} start={start}", flush=True)
+            print(f"[filter philox] filter={filt} W={W} S={S} L={L} start={start}", flush=True)
             g0 = _start(start, R, L, seed)
             got, mag = run(L, R, g0, seed, False, beta, chunks)
             ref = O.sweep_philox(L, g0.copy(), seed, 0, sum(chunks) * L * L, beta, H)
@@ -172,12 +175,18 @@ def test_conflict_filter_choice(gpu, engine_lib):
     try:
         G.set_conflict_filter(2)
         assert (filt(90, 64), filt(91, 64)) == (2, 1)
+        # eighth words: a byte per site, so the plane fits up to L = 181
+        assert (filt(181, 64, S=16, W=8), filt(182, 64, S=16, W=8)) == (2, 1)
+        assert filt(64, 64, S=4, W=4) == 1  # quarter and eighth words from 8 slots
         G.set_conflict_filter(0)
         assert filt(64, 16384) == 2
         assert filt(64, 524288, S=8) == 1
         assert filt(64, 64, S=16, W=1) == 1
         assert filt(64, 64, S=1) == 1
         assert filt(64, 64, method=1, S=8, W=1) == 1
+        assert filt(64, 2048, S=16, W=4) == 1  # quarter words at L <= 64 below 32 slots: masks
+        assert filt(64, 2048, S=32, W=4) == 2
+        assert filt(96, 2048, S=16, W=8) == 2
     finally:
         G.set_conflict_filter(0)
 
