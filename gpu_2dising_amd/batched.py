@@ -57,29 +57,39 @@ class Batch:
 
     def __init__(self, groups=CONFIG4_GROUPS, chk: int = 20, seed: int = 2026, tune: bool = True,
                  probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9, shapes=None,
-                 conflict_filter: int = 1):
+                 conflict_filter=None):
         """`shapes`: optional (slots, waves, sweeps per launch) per group, used
         instead of the tuner's choice (for comparing builds at fixed shapes).
         `conflict_filter`: the method-2 filter of the groups' engines (as
-        set_conflict_filter takes it; the process setting returns to automatic
-        afterwards).  The row/column masks by default: the mark plane doubles a
-        half-word workgroup's LDS (more at small L), and in the graph fewer of
-        those workgroups then fit beside the large-L group's; at the batched
-        configuration's shapes replays took 6.3-7.6 ms with it against
-        5.3-5.7 ms without, although each group alone ran faster with it."""
-        set_conflict_filter(conflict_filter)
+        set_conflict_filter takes it), one for all or one per group; the
+        process setting returns to automatic afterwards.  By default the
+        shapes are tuned with the row/column masks, and only half-word groups
+        whose mark plane is at most 4 KB (L <= 32) then take the plane: it
+        doubles a workgroup's LDS, and in the graph fewer workgroups of the
+        larger groups fit beside the large-L group's.  At the batched
+        configuration's shapes replays took 5.16-5.24 ms with the plane for
+        L = 32 only, 5.15-5.50 with none, 5.7-6.0 with L = 48 too, 6.5-7.1
+        with L = 96 and 6.1-6.4 with every method-2 group on it, although
+        each group alone ran faster with it."""
+        auto = conflict_filter is None
+        filters = ([1] * len(groups) if auto else
+                   list(conflict_filter) if isinstance(conflict_filter, (list, tuple))
+                   else [conflict_filter] * len(groups))
+        if len(filters) != len(groups):
+            raise ValueError("conflict_filter: one value, or one per group")
         try:
-            self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes)
+            self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto)
         finally:
             set_conflict_filter(0)
 
-    def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes):
+    def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto):
         rs = np.random.default_rng(seed)
         self.chk, self.dn, self.up = chk, dn, up
         self.groups: list[Group] = []
         for k, (L, R, m) in enumerate(groups):
             beta, h = float(rs.uniform(0.50, 0.60)), float(rs.uniform(0.03, 0.10))
             g = Group(L=L, R=R, method=m, beta=beta, h=h, itask=k % 2, seed=seed + k)
+            set_conflict_filter(filters[k])
             if shapes is not None:
                 g.slots, g.waves, g.launch_sweeps = shapes[k]
             elif tune:
@@ -87,7 +97,11 @@ class Batch:
             else:
                 g.launch_sweeps = chk
             self.groups.append(g)
-        for g in self.groups:
+        if auto:  # the mark plane for half-word groups whose plane is at most 4 KB
+            filters = [0 if g.method == 2 and g.waves == 2 and 4 * g.L * g.L <= 4096 else 1
+                       for g in self.groups]
+        for g, f in zip(self.groups, filters):
+            set_conflict_filter(f)
             g.rng = RngState(g.R, g.seed)
             if shapes is not None:
                 set_tuning(g.slots, g.waves)
