@@ -487,7 +487,7 @@ def set_lattice_staging(staging: int = 0) -> None:
 
 
 def set_conflict_filter(filter: int = 0) -> None:
-    """Conflict filter of method 2 on half words: 1 row/column masks, 2 mark plane (0 = automatic)."""
+    """Conflict filter of method 2 on split words: 1 row/column masks, 2 mark plane (0 = automatic)."""
     lib().gasp_set_conflict_filter(filter)
 
 
