@@ -263,11 +263,14 @@ def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
     256; beta ~ U[0.50, 0.60], h ~ U[0.03, 0.10] per group; itask 0 (count
     nucleated) and itask 1 (fates from droplet inputs) alternate.  Launch
     shapes and sweeps per launch come from the autotuner, probed at each
-    group's own (beta, h).  One replay = `chk` sweeps of every group plus its
+    group's own (beta, h), then refined by a coordinate search on the captured
+    graph (`Batch._refine`).  One replay = `chk` sweeps of every group plus its
     threshold check."""
     from gpu_2dising_amd.batched import Batch
 
-    b = Batch(chk=chk, seed=seed)
+    t0 = time.perf_counter()
+    b = Batch(chk=chk, seed=seed, refine=True)
+    setup_s = time.perf_counter() - t0
     attempts = b.attempts_per_replay
     graph = b.graph
     graph.launch(1)  # warm-up
@@ -287,12 +290,14 @@ def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
                "sweeps_per_launch": g.launch_sweeps, "probe_rate": g.probe_rate, "beta": g.beta, "h": g.h,
                "itask": g.itask} for g in b.groups]
     order = [b.groups[i].L for i in b.order]
+    tuned_ms, refine_log = b.tuned_ms, b.refine_log
     b.close()
     return {"metric": "batched spin-flip attempts/sec (one captured graph, uneven L)",
             "value": attempts / (g_ms * 1e-3), "unit": "attempts/s",
             "wall_value": attempts * replays / wall, "graph_ms_per_replay": g_ms,
             "sequential_ms_per_replay": seq_ms, "sweeps_per_replay": chk, "replays": replays,
-            "attempts_per_replay": attempts, "capture_order_L": order, "groups": groups}
+            "attempts_per_replay": attempts, "capture_order_L": order, "groups": groups,
+            "tuned_ms_per_replay": tuned_ms, "refinement": refine_log, "setup_seconds": setup_s}
 
 
 def fit_rate(sweeps: np.ndarray, frac: np.ndarray):

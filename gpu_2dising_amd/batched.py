@@ -57,7 +57,7 @@ class Batch:
 
     def __init__(self, groups=CONFIG4_GROUPS, chk: int = 20, seed: int = 2026, tune: bool = True,
                  probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9, shapes=None,
-                 conflict_filter=None):
+                 conflict_filter=None, refine: bool = False, refine_replays: int = 10):
         """`shapes`: optional (slots, waves, sweeps per launch) per group, used
         instead of the tuner's choice (for comparing builds at fixed shapes).
         `conflict_filter`: the method-2 filter of the groups' engines (as
@@ -70,17 +70,24 @@ class Batch:
         configuration's shapes replays took 5.16-5.24 ms with the plane for
         L = 32 only, 5.15-5.50 with none, 5.7-6.0 with L = 48 too, 6.5-7.1
         with L = 96 and 6.1-6.4 with every method-2 group on it, although
-        each group alone ran faster with it."""
+        each group alone ran faster with it.
+        `refine`: after the per-group tuning, a coordinate search over the
+        groups' shapes on the captured graph itself (see `_refine`)."""
         auto = conflict_filter is None
         filters = ([1] * len(groups) if auto else
                    list(conflict_filter) if isinstance(conflict_filter, (list, tuple))
                    else [conflict_filter] * len(groups))
         if len(filters) != len(groups):
             raise ValueError("conflict_filter: one value, or one per group")
+        self.auto_filter = auto
+        self.refine_log, self.tuned_ms, self.refined_ms = [], None, None
         try:
             self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto)
+            if refine and shapes is None:
+                self._refine(refine_replays)
         finally:
             set_conflict_filter(0)
+            set_tuning(0, 0)
 
     def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto):
         rs = np.random.default_rng(seed)
@@ -97,30 +104,11 @@ class Batch:
             else:
                 g.launch_sweeps = chk
             self.groups.append(g)
-        if auto:  # the mark plane for half-word groups whose plane is at most 4 KB
-            filters = [0 if g.method == 2 and g.waves == 2 and 4 * g.L * g.L <= 4096 else 1
-                       for g in self.groups]
-        for g, f in zip(self.groups, filters):
-            set_conflict_filter(f)
-            g.rng = RngState(g.R, g.seed)
-            if shapes is not None:
-                set_tuning(g.slots, g.waves)
-            try:
-                g.engine = Engine(g.L, g.R, g.method, g.rng, g.beta, g.h)
-            finally:
-                if shapes is not None:
-                    set_tuning(0, 0)
+        self.filters = filters
+        for k, g in enumerate(self.groups):
             if shapes is not None:
                 g.probe_rate = 1.0  # capture order by the groups' attempts
-            if g.itask == 0:
-                g.start = -np.ones((g.R, g.L, g.L), dtype=np.int32)
-                g.engine.set_grids(g.start)
-            else:
-                ins = droplet_inputs(g.L)
-                g.ninputs = ins.shape[0]
-                g.start = np.repeat(ins, g.R // g.ninputs, axis=0)
-                g.engine.set_inputs(ins)
-                g.engine.set_fates(np.full(g.R, -1, dtype=np.int32))
+            self._make_engine(k, fixed=shapes is not None)
         # branches captured longest first (each group's time alone estimated
         # from its tuner probe rate), so that the long groups start before the
         # short ones fill the CUs; except that an HBM-path group whose replicas
@@ -133,11 +121,116 @@ class Batch:
         if len(order) > 1 and big_lds[order[0]] and not big_lds[order[1]]:
             order[0], order[1] = order[1], order[0]
         self.order = order
-        gs = [self.groups[i] for i in order]
-        self.graph = Graph([g.engine for g in gs], [chk] * len(gs),
+        self._capture()
+
+    def _filter(self, k: int) -> int:
+        g = self.groups[k]
+        if self.auto_filter:  # the mark plane for half-word groups whose plane is at most 4 KB
+            return 0 if g.method == 2 and g.waves == 2 and 4 * g.L * g.L <= 4096 else 1
+        return self.filters[k]
+
+    def _make_engine(self, k: int, fixed: bool) -> None:
+        """Group k's RNG streams (from attempt 0) and engine, at its shape if
+        `fixed` (else the tuner's), loaded with its start state."""
+        g = self.groups[k]
+        set_conflict_filter(self._filter(k))
+        g.rng = RngState(g.R, g.seed)
+        if fixed:
+            set_tuning(g.slots, g.waves)
+        try:
+            g.engine = Engine(g.L, g.R, g.method, g.rng, g.beta, g.h)
+        finally:
+            set_tuning(0, 0)
+        if g.itask == 0:
+            g.start = -np.ones((g.R, g.L, g.L), dtype=np.int32)
+        else:
+            ins = droplet_inputs(g.L)
+            g.ninputs = ins.shape[0]
+            g.start = np.repeat(ins, g.R // g.ninputs, axis=0)
+        self._load_start(g)
+
+    @staticmethod
+    def _load_start(g: Group) -> None:
+        if g.itask == 0:
+            g.engine.set_grids(g.start)
+        else:
+            g.engine.set_inputs(droplet_inputs(g.L))
+            g.engine.set_fates(np.full(g.R, -1, dtype=np.int32))
+
+    def _capture(self) -> None:
+        gs = [self.groups[i] for i in self.order]
+        self.graph = Graph([g.engine for g in gs], [self.chk] * len(gs),
                            [CHECK_COUNT if g.itask == 0 else CHECK_FATES for g in gs],
-                           [dn] * len(gs), [up] * len(gs),
-                           [chk // max(g.launch_sweeps, 1) for g in gs])
+                           [self.dn] * len(gs), [self.up] * len(gs),
+                           [self.chk // max(g.launch_sweeps, 1) for g in gs])
+
+    def _time_replay(self, replays: int) -> float:
+        """ms per replay from the groups' start states (replays grow slower as
+        the lattices nucleate), after one untimed replay."""
+        for g in self.groups:
+            self._load_start(g)
+            g.engine.sync()
+        self.graph.launch(1)
+        ms0 = self.graph.ms()
+        self.graph.launch(replays)
+        return (self.graph.ms() - ms0) / replays
+
+    def _candidates(self, g: Group) -> list:
+        if g.method == 0 and g.slots > 64:
+            return [(s_, s_ // 64, k) for s_ in (256, 512) for k in (self.chk // 4, self.chk // 2, self.chk)
+                    if k >= 1 and self.chk % k == 0]
+        if g.method != 2:
+            return []
+        slots = [s_ for s_ in (g.slots // 2, g.slots, 2 * g.slots) if 2 <= s_ <= 32]
+        return [(s_, w_, self.chk) for s_ in slots for w_ in (2, 4, 8)]
+
+    def _refine(self, replays: int, tol: float = 0.02) -> None:
+        """Coordinate search over the groups' shapes on the captured graph: the
+        tuner probes each group alone, where wide speculation fills the device,
+        while in the graph the groups share it.  For each group in capture
+        order, each candidate (method 2: half, the same and twice the slots on
+        half, quarter or eighth words, all sweeps in one launch; the multi-wave
+        method 0: 4 or 8 waves, launches of a quarter, half or all of the
+        sweeps) is timed over `replays` replays from the start states and kept
+        if it beats the best so far by `tol` ms.  The searches advance the RNG
+        streams, so every group is then rebuilt from attempt 0.  At the batched
+        configuration one pass took the replay from 5.15 to 4.78 ms (L = 32 on
+        8 slots, L = 48 on quarter words with 16)."""
+        best = self._time_replay(replays)
+        self.tuned_ms = best
+        for k in list(self.order):
+            g = self.groups[k]
+            for cand in self._candidates(g):
+                old = (g.slots, g.waves, g.launch_sweeps)
+                if cand == old:
+                    continue
+                g.slots, g.waves, g.launch_sweeps = cand
+                try:
+                    self._rebuild(k)
+                    t = self._time_replay(replays)
+                except RuntimeError:
+                    t = None
+                if t is not None and t < best - tol:
+                    self.refine_log.append({"L": g.L, "from": list(old), "to": list(cand), "ms": t})
+                    best = t
+                else:
+                    g.slots, g.waves, g.launch_sweeps = old
+                    self._rebuild(k)
+        self.graph.close()
+        for k, g in enumerate(self.groups):
+            g.engine.close()
+            g.rng.free()
+            self._make_engine(k, fixed=True)
+        self._capture()
+        self.refined_ms = best
+
+    def _rebuild(self, k: int) -> None:
+        g = self.groups[k]
+        self.graph.close()
+        g.engine.close()
+        g.rng.free()
+        self._make_engine(k, fixed=True)
+        self._capture()
 
     @property
     def attempts_per_replay(self) -> int:

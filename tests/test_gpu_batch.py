@@ -180,6 +180,34 @@ def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
     b.close()
 
 
+def test_batched_refined_graph(gpu, engine_lib, oracle):
+    """Batch with the coordinate search on the captured graph: the chosen
+    shapes are valid and no slower than the tuner's (by the search's own
+    timings), and after the search every group starts again from attempt 0
+    of its streams: two replays, then for 6 replicas of every group the
+    oracle replays the same Philox streams from the start state."""
+    from gpu_2dising_amd.batched import Batch
+
+    G = engine_lib
+    print("[batched] refined on the graph", flush=True)
+    b = Batch(chk=20, seed=2026, refine=True, refine_replays=2)
+    try:
+        assert b.tuned_ms is not None and b.refined_ms <= b.tuned_ms
+        for g in b.groups:
+            assert g.engine.shape() == (g.slots, g.waves), (g.L, g.engine.shape(), g.slots, g.waves)
+        b.graph.launch(2)
+        rs = np.random.default_rng(11)
+        for g in b.groups:
+            print(f"[batched refined] L={g.L} shape=({g.slots}, {g.waves}, {g.launch_sweeps})", flush=True)
+            idx = np.unique(np.concatenate([[0, g.R - 1], rs.choice(g.R, 4, replace=False)]))
+            grids = g.start[idx].copy()
+            N = g.L * g.L
+            oracle.sweep_philox_replicas(g.L, grids, idx, g.seed, 0, 2 * b.chk * N, g.beta, g.h)
+            assert np.array_equal(g.engine.gather(idx).astype(np.int32), grids), g.L
+    finally:
+        b.close()
+
+
 def test_batched_fixed_shapes(gpu, engine_lib, oracle):
     """Batch with explicit (slots, waves, sweeps per launch) per group instead
     of the tuner's: each engine runs at the given shape, and its replicas
