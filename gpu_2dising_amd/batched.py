@@ -184,7 +184,7 @@ class Batch:
         slots = [s_ for s_ in (g.slots // 2, g.slots, 2 * g.slots) if 2 <= s_ <= 32]
         return [(s_, w_, self.chk) for s_ in slots for w_ in (2, 4, 8)]
 
-    def _refine(self, replays: int, tol: float = 0.02) -> None:
+    def _refine(self, replays: int, tol: float = 0.02, passes: int = 2) -> None:
         """Coordinate search over the groups' shapes on the captured graph: the
         tuner probes each group alone, where wide speculation fills the device,
         while in the graph the groups share it.  For each group in capture
@@ -192,30 +192,37 @@ class Batch:
         half, quarter or eighth words, all sweeps in one launch; the multi-wave
         method 0: 4 or 8 waves, launches of a quarter, half or all of the
         sweeps) is timed over `replays` replays from the start states and kept
-        if it beats the best so far by `tol` ms.  The searches advance the RNG
-        streams, so every group is then rebuilt from attempt 0.  At the batched
-        configuration one pass took the replay from 5.15 to 4.78 ms (L = 32 on
-        8 slots, L = 48 on quarter words with 16)."""
+        if it beats the best so far by `tol` ms; up to `passes` passes, until
+        one changes nothing (the groups' choices interact: a second pass
+        moved the L = 256 group back to eight waves).  The searches advance
+        the RNG streams, so every group is then rebuilt from attempt 0.  At
+        the batched configuration the replay went from 5.16 to 4.92 ms in one
+        pass and to 4.76-4.78 in two (L = 32 on 8 slots, L = 48 on quarter
+        words with 16)."""
         best = self._time_replay(replays)
         self.tuned_ms = best
-        for k in list(self.order):
-            g = self.groups[k]
-            for cand in self._candidates(g):
-                old = (g.slots, g.waves, g.launch_sweeps)
-                if cand == old:
-                    continue
-                g.slots, g.waves, g.launch_sweeps = cand
-                try:
-                    self._rebuild(k)
-                    t = self._time_replay(replays)
-                except RuntimeError:
-                    t = None
-                if t is not None and t < best - tol:
-                    self.refine_log.append({"L": g.L, "from": list(old), "to": list(cand), "ms": t})
-                    best = t
-                else:
-                    g.slots, g.waves, g.launch_sweeps = old
-                    self._rebuild(k)
+        for _ in range(passes):
+            changed = False
+            for k in list(self.order):
+                g = self.groups[k]
+                for cand in self._candidates(g):
+                    old = (g.slots, g.waves, g.launch_sweeps)
+                    if cand == old:
+                        continue
+                    g.slots, g.waves, g.launch_sweeps = cand
+                    try:
+                        self._rebuild(k)
+                        t = self._time_replay(replays)
+                    except RuntimeError:
+                        t = None
+                    if t is not None and t < best - tol:
+                        self.refine_log.append({"L": g.L, "from": list(old), "to": list(cand), "ms": t})
+                        best, changed = t, True
+                    else:
+                        g.slots, g.waves, g.launch_sweeps = old
+                        self._rebuild(k)
+            if not changed:
+                break
         self.graph.close()
         for k, g in enumerate(self.groups):
             g.engine.close()
