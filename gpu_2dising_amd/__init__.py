@@ -31,6 +31,7 @@ EXPORTS = (
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats", "gasp_set_conflict_filter",
     "gasp_engine_get_conflict_filter", "gasp_set_lattice_staging", "gasp_engine_get_lattice_staging",
+    "gasp_engine_set_attempt",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
     "gasp_engine_gather_grids", "gasp_engine_get_gridstates",
     "gasp_engine_set_inputs", "gasp_engine_set_freeze",
@@ -162,6 +163,7 @@ def lib() -> C.CDLL:
     L.gasp_engine_get_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.gasp_engine_get_conflict_filter.argtypes = [vp, C.POINTER(C.c_int)]
     L.gasp_engine_get_lattice_staging.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gasp_engine_set_attempt.argtypes = [vp, C.c_ulonglong]
     L.gasp_engine_stream.argtypes = [vp]
     L.gasp_engine_stream.restype = vp
     L.gasp_sweep_grids.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, C.c_int, vp, vp]
@@ -316,6 +318,10 @@ class Engine:
 
     def kernel_ms(self) -> float:
         return lib().gasp_engine_kernel_ms(self.ptr)
+
+    def set_attempt(self, attempt: int) -> None:
+        """Move the replicas to attempt `attempt` of their Philox streams (rewind or skip)."""
+        check(lib().gasp_engine_set_attempt(self.ptr, attempt))
 
     def shape(self) -> tuple:
         """(slots per replica, waves per workgroup or word split) of the engine's launches."""

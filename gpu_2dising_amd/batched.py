@@ -194,12 +194,15 @@ class Batch:
         sweeps) is timed over `replays` replays from the start states and kept
         if it beats the best so far by `tol` ms; up to `passes` passes, until
         one changes nothing (the groups' choices interact: a second pass
-        moved the L = 256 group back to eight waves).  The searches advance
-        the RNG streams, so every group is then rebuilt from attempt 0.  At
+        moved the L = 256 group back to eight waves).  Each timing uses a
+        checked capture (`_capture_checked`).  The searches advance the RNG
+        streams and the lattices, so every group ends at attempt 0 of its
+        streams and at its start state.  At
         the batched configuration the replay went from 5.16 to 4.92 ms in one
         pass and to 4.76-4.78 in two (L = 32 on 8 slots, L = 48 on quarter
         words with 16)."""
-        best = self._time_replay(replays)
+        self.graph.close()
+        best = self._capture_checked(replays)
         self.tuned_ms = best
         for _ in range(passes):
             changed = False
@@ -212,7 +215,8 @@ class Batch:
                     g.slots, g.waves, g.launch_sweeps = cand
                     try:
                         self._rebuild(k)
-                        t = self._time_replay(replays)
+                        # one recapture at most: most candidates that look slow are
+                        t = self._capture_checked(replays, ref=best, tries=2)
                     except RuntimeError:
                         t = None
                     if t is not None and t < best - tol:
@@ -221,23 +225,56 @@ class Batch:
                     else:
                         g.slots, g.waves, g.launch_sweeps = old
                         self._rebuild(k)
+                        self._capture()
             if not changed:
                 break
+        # fresh engines at the chosen shapes, an instance of the graph that
+        # replays at the searched speed, then every group back to attempt 0
+        # and its start state
         self.graph.close()
         for k, g in enumerate(self.groups):
             g.engine.close()
             g.rng.free()
             self._make_engine(k, fixed=True)
+        self.refined_ms = self._capture_checked(replays, ref=best)
+        for g in self.groups:
+            g.engine.set_attempt(0)
+            self._load_start(g)
+
+    def _capture_checked(self, replays: int, ref: float = None, tries: int = 4) -> float:
+        """Captures the graph and times it from the start states.  Instances of
+        one graph replay at one of two speeds (at the batched configuration
+        4.75 or 7.3 ms, about one capture in five the slower; with 8 hardware
+        queues 5.1 or 5.8 ms), so an instance more than 20% slower than `ref`
+        (without `ref`: than another capture) is captured again, up to
+        `tries` captures.  Without `ref` the reference is the fastest of three
+        captures (two slow instances in a row are not rare enough).  Returns
+        the kept instance's ms per replay."""
         self._capture()
-        self.refined_ms = best
+        t = self._time_replay(replays)
+        if ref is None:
+            ref = t
+            for _ in range(2):
+                self.graph.close()
+                self._capture()
+                t = self._time_replay(replays)
+                ref = min(ref, t)
+        for _ in range(tries - 1):
+            if t <= 1.2 * ref:
+                break
+            self.graph.close()
+            self._capture()
+            t = self._time_replay(replays)
+        return t
 
     def _rebuild(self, k: int) -> None:
+        """Group k's engine again at its current shape (the graph is closed,
+        for the caller to capture again)."""
         g = self.groups[k]
         self.graph.close()
         g.engine.close()
         g.rng.free()
         self._make_engine(k, fixed=True)
-        self._capture()
 
     @property
     def attempts_per_replay(self) -> int:

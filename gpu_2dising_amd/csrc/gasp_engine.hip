@@ -1257,6 +1257,18 @@ extern "C" double gasp_engine_kernel_ms(gasp_engine *e) {
 
 extern "C" void *gasp_engine_stream(gasp_engine *e) { return e ? (void *)e->stream : nullptr; }
 
+extern "C" int gasp_engine_set_attempt(gasp_engine *e, unsigned long long attempt) {
+    clear_error();
+    if (!e) return set_error(GASP_ERR_ARG, "gasp_engine_set_attempt: NULL engine");
+    if (e->rng.kind != KIND_PHILOX)
+        return set_error(GASP_ERR_ARG, "gasp_engine_set_attempt: only counter-based (Philox) streams rewind");
+    const int rc = engine_sync(e);  // launches in flight used the old position
+    if (rc) return rc;
+    e->rng.attempts = (uint64_t)attempt;
+    HIP_TRY(hipMemcpy(e->d_rng, &e->rng, sizeof(RngHeader), hipMemcpyHostToDevice));
+    return GASP_OK;
+}
+
 extern "C" int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves) {
     if (!e || !slots || !waves) return set_error(GASP_ERR_ARG, "gasp_engine_get_shape: NULL argument");
     *slots = e->S;

@@ -353,6 +353,12 @@ void *gasp_engine_stream(gasp_engine *e);
 /* The engine's launch shape: slots per replica and waves per workgroup (the
  * word split for method 2), as gasp_set_tuning takes them. */
 int gasp_engine_get_shape(gasp_engine *e, int *slots, int *waves);
+/* Moves the engine's replicas to per-replica attempt `attempt` of their
+ * Philox streams (the next sweep uses attempts attempt, attempt + 1, ...):
+ * rewinding replays a trajectory's random numbers.  Waits for the engine's
+ * launches.  Counter-based streams only (MT19937 states are refused).  No
+ * counterpart in the reference (curand states do not rewind). */
+int gasp_engine_set_attempt(gasp_engine *e, unsigned long long attempt);
 /* The engine's conflict filter, as gasp_set_conflict_filter takes it (1 for
  * engines without the choice). */
 int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter);

@@ -191,6 +191,38 @@ def test_conflict_filter_choice(gpu, engine_lib):
         G.set_conflict_filter(0)
 
 
+def test_set_attempt_rewinds_streams(gpu, engine_lib, oracle):
+    """gasp_engine_set_attempt: after sweeps, rewinding to attempt 0 and
+    reloading the start grids replays the same trajectory; moving to attempt
+    n continues the oracle's stream from there; MT19937 states are refused."""
+    G, O = engine_lib, oracle
+    L, R, seed = 16, 64, 4321
+    g0 = _start("random", R, L, seed)
+    rng = G.RngState(R, seed)
+    try:
+        with G.Engine(L, R, 2, rng, 0.3, H) as e:
+            e.set_grids(g0)
+            e.sweep(3)
+            first = e.get_grids()
+            e.set_attempt(0)
+            e.set_grids(g0)
+            e.sweep(3)
+            assert np.array_equal(e.get_grids(), first)
+            e.set_attempt(7 * L * L)
+            e.set_grids(g0)
+            e.sweep(2)
+            assert np.array_equal(e.get_grids(), O.sweep_philox(L, g0.copy(), seed, 7 * L * L, 2 * L * L, 0.3, H))
+    finally:
+        rng.free()
+    mt = G.RngState(R, seed, mt=True)
+    try:
+        with G.Engine(L, R, 2, mt, 0.3, H) as e:
+            with pytest.raises(G.GaspError):
+                e.set_attempt(0)
+    finally:
+        mt.free()
+
+
 @pytest.mark.parametrize("staging", [1, 2])
 @pytest.mark.parametrize("S", [128, 256, 512, 1024])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S, staging):
