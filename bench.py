@@ -397,7 +397,7 @@ def cpu_baseline(max_seconds: float = 12.0) -> dict:
                              str(L)], cwd=ROOT, capture_output=True, text=True, timeout=120)
         if r1.returncode == 0:
             ref["one_core"] = json.loads(r1.stdout.strip().splitlines()[-1])
-        # the reference's literal single-stream driver (one MT19937 over all replicas), config 1's shape
+        # the reference's literal single-stream driver (one MT19937 over all replicas) at L = 32
         r2 = subprocess.run([sys.executable, "-m", "oracle.refbench", "--driver", "--sweeps", "1000"], cwd=ROOT,
                             capture_output=True, text=True, timeout=180)
         if r2.returncode == 0:

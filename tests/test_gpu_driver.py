@@ -1,6 +1,6 @@
 """mc_driver_gpu through the C ABI against the oracle's restatement of the
 reference schedule (src/mc_gpu.cu:530-831) and against the reference's own
-golden vectors (computed by the compiled src/mc_cpu.c).
+golden vectors (L = 32, 256 replicas, 2000 sweeps, computed by the compiled src/mc_cpu.c).
 
 Observable outputs compared exactly: the result array (nucleated fraction or
 pB), the fates, and everything handed to the output callback (sweep number,
@@ -112,8 +112,9 @@ def test_cluster_cv_matches_oracle(gpu, engine_lib, oracle):
 
 
 @pytest.mark.parametrize("method", [1, 2])
-def test_config1_against_reference_goldens(gpu, engine_lib, oracle, method):
-    """BASELINE config 1 through mc_driver_gpu with per-replica MT19937 streams:
+def test_plain_run_against_reference_goldens(gpu, engine_lib, oracle, method):
+    """The plain reference run (L = 32, 256 replicas, 2000 sweeps, seeds
+    base + r) through mc_driver_gpu with per-replica MT19937 streams:
     the magnetisation at every check (handed to the output callback) and the final
     grids equal the reference's compiled mc_cpu.c, bit for bit."""
     gold = np.load(GOLD)

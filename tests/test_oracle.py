@@ -188,8 +188,8 @@ def test_sweep_matches_reference_build(oracle, gold, L):
     assert np.array_equal(got, ref)
 
 
-def test_config1_matches_reference_build(oracle, gold):
-    """BASELINE config 1 (L=32, 256 replicas, 2000 sweeps, seeds base+r):
+def test_plain_run_matches_reference_build(oracle, gold):
+    """The plain reference run (L=32, 256 replicas, 2000 sweeps, seeds base+r):
     magnetisation at every check and the final grids, bit for bit."""
     L, nrep, chunks, spc, seed = (int(x) for x in gold["c1_meta"])
     g = -np.ones((nrep, L, L), dtype=np.int32)
