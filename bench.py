@@ -227,7 +227,7 @@ def surrogate_leg(G, shooting: dict, n_inputs: int = 512, fallback_reps: int = 1
            "unit": "pB estimates/s", "inputs": n_inputs, "band_inputs": int(band.sum()),
            "reference": "pB of the shooting leg (1024 replicas per input, the same 512 inputs)",
            "plain_shooting": plain}
-    for name, path in (("mixed", SURROGATE_PARAMS), ("config5", SURROGATE_PARAMS_TRAJ)):
+    for name, path in (("mixed", SURROGATE_PARAMS), ("trajectory", SURROGATE_PARAMS_TRAJ)):
         if not os.path.exists(path):
             continue
         blob = np.load(path)

@@ -11,7 +11,7 @@ total attempts over the slowest process's time.
   python3 -m oracle.refbench --cores 16 --seconds 12 --L 64
 prints one JSON object.  --driver instead times the reference's literal
 single-stream schedule, src/mc_cpu.c:106-308 mc_driver_cpu (one MT19937 stream
-over all replicas, checks every 100 sweeps), on config 1's shape (L = 32, 256
+over all replicas, checks every 100 sweeps), on the plain run's shape (L = 32, 256
 replicas, beta = 0.54, h = 0.07, all -1) for a bounded number of sweeps.
 """
 from __future__ import annotations
@@ -186,7 +186,7 @@ def run_committor(cores: int, replicas: int = 64, max_sweeps: int = 20000, seed:
 
 
 def run_driver(nsweeps: int = 1000) -> dict:
-    """src/mc_cpu.c mc_driver_cpu on config 1's shape, in a forked child."""
+    """src/mc_cpu.c mc_driver_cpu at L = 32, 256 replicas, in a forked child."""
     from oracle import oracle as O
 
     if O.ref() is None:
@@ -208,7 +208,7 @@ def main() -> int:
     ap.add_argument("--cores", type=int, default=1)
     ap.add_argument("--seconds", type=float, default=12.0)
     ap.add_argument("--L", type=int, default=64)
-    ap.add_argument("--driver", action="store_true", help="time mc_driver_cpu on config 1's shape")
+    ap.add_argument("--driver", action="store_true", help="time mc_driver_cpu at L = 32, 256 replicas")
     ap.add_argument("--sweeps", type=int, default=1000, help="--driver: sweeps")
     ap.add_argument("--committor", action="store_true",
                     help="time mc_driver_cpu itask 1 on committor inputs 0..cores-1")

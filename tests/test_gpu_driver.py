@@ -119,7 +119,7 @@ def test_plain_run_against_reference_goldens(gpu, engine_lib, oracle, method):
     grids equal the reference's compiled mc_cpu.c, bit for bit."""
     gold = np.load(GOLD)
     L, nrep, chunks, spc, seed = (int(x) for x in gold["c1_meta"])
-    print(f"[config1] method={method}", flush=True)
+    print(f"[plain run L=32] method={method}", flush=True)
     g0 = -np.ones((nrep, L, L), dtype=np.int32)
     out, cap = _run(engine_lib, L, g0, mt=True, seed=seed, method=method, tot=chunks * spc, mag=spc,
                     gout=spc)
