@@ -103,7 +103,7 @@ def test_split_words_at_L64(gpu, engine_lib, oracle):
 
 
 @pytest.mark.parametrize("filt", [1, 2])
-@pytest.mark.parametrize("W,S", [(2, 2), (2, 4), (2, 16), (2, 32), (4, 8), (4, 16), (8, 8), (8, 32)])
+@pytest.mark.parametrize("W,S", [(2, 2), (2, 4), (2, 8), (2, 16), (2, 32), (4, 8), (4, 16), (4, 32), (8, 8), (8, 16), (8, 32)])
 def test_conflict_filters_bit_exact(gpu, engine_lib, oracle, filt, W, S):
     """Method 2 on split words (W = 2, 4, 8: half, quarter, eighth words)
     with either conflict filter (1 row/column masks, 2 the mark plane; on
