@@ -330,7 +330,7 @@ class Engine:
         return s.value, w.value
 
     def lattice_staging(self) -> int:
-        """The engine's lattice staging: 1 in HBM, 2 staged in LDS (method 0 over several waves)."""
+        """The engine's lattice staging: 1 in HBM, 2 staged in LDS as bytes, 3 as bits (method 0 over several waves)."""
         f = C.c_int(0)
         check(lib().gasp_engine_get_lattice_staging(self.ptr, C.byref(f)))
         return f.value
@@ -488,7 +488,7 @@ def set_tuning(slots: int = 0, waves: int = 0) -> None:
 
 
 def set_lattice_staging(staging: int = 0) -> None:
-    """Method 0 over several waves: 1 lattice in HBM, 2 staged in LDS (0 = automatic)."""
+    """Method 0 over several waves: 1 lattice in HBM, 2 staged in LDS as bytes, 3 as bits (0 = automatic)."""
     lib().gasp_set_lattice_staging(staging)
 
 

@@ -12,7 +12,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import CHECK_COUNT, CHECK_FATES, Engine, Graph, RngState, autotune, set_conflict_filter, set_tuning
+from . import (CHECK_COUNT, CHECK_FATES, Engine, Graph, RngState, autotune, set_conflict_filter,
+               set_lattice_staging, set_tuning)
 
 # L, replicas, gpu_method: four lattice sizes on the LDS multispin path and
 # one on the HBM byte lattice
@@ -57,7 +58,8 @@ class Batch:
 
     def __init__(self, groups=CONFIG4_GROUPS, chk: int = 20, seed: int = 2026, tune: bool = True,
                  probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9, shapes=None,
-                 conflict_filter=None, refine: bool = False, refine_replays: int = 10):
+                 conflict_filter=None, refine: bool = False, refine_replays: int = 10,
+                 lattice_staging: int = 2):
         """`shapes`: optional (slots, waves, sweeps per launch) per group, used
         instead of the tuner's choice (for comparing builds at fixed shapes).
         `conflict_filter`: the method-2 filter of the groups' engines (as
@@ -72,7 +74,15 @@ class Batch:
         with L = 96 and 6.1-6.4 with every method-2 group on it, although
         each group alone ran faster with it.
         `refine`: after the per-group tuning, a coordinate search over the
-        groups' shapes on the captured graph itself (see `_refine`)."""
+        groups' shapes on the captured graph itself (see `_refine`).
+        `lattice_staging`: the staging of multi-wave method-0 groups (as
+        set_lattice_staging takes it) for the tuner's probes and the engines;
+        by default bytes in LDS.  Alone, L = 256 with 256 replicas runs 3%
+        faster staged as bits (the engine's automatic choice there: its
+        workgroup keeps 16384-entry dependence tables in 140 KB of LDS), but
+        the batched configuration's refined graph replayed in 5.57 ms that
+        way against 4.77 with bytes (8192 entries, 129 KB), and in 4.91 ms with
+        bits beside 8192 entries."""
         auto = conflict_filter is None
         filters = ([1] * len(groups) if auto else
                    list(conflict_filter) if isinstance(conflict_filter, (list, tuple))
@@ -80,13 +90,16 @@ class Batch:
         if len(filters) != len(groups):
             raise ValueError("conflict_filter: one value, or one per group")
         self.auto_filter = auto
+        self.staging = lattice_staging
         self.refine_log, self.tuned_ms, self.refined_ms = [], None, None
+        set_lattice_staging(lattice_staging)
         try:
             self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto)
             if refine and shapes is None:
                 self._refine(refine_replays)
         finally:
             set_conflict_filter(0)
+            set_lattice_staging(0)
             set_tuning(0, 0)
 
     def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto):
@@ -134,6 +147,7 @@ class Batch:
         `fixed` (else the tuner's), loaded with its start state."""
         g = self.groups[k]
         set_conflict_filter(self._filter(k))
+        set_lattice_staging(self.staging)
         g.rng = RngState(g.R, g.seed)
         if fixed:
             set_tuning(g.slots, g.waves)

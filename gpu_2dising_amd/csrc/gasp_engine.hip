@@ -98,7 +98,7 @@ uint64_t g_thr_f32[10];
 int g_tune_slots = 0;
 int g_tune_waves = 0;
 int g_tune_filter = 0;  // method 2 on half words: 1 row/column masks, 2 mark plane (0: automatic)
-int g_tune_stage = 0;   // method 0 over several waves: 1 lattice in HBM, 2 staged in LDS (0: automatic)
+int g_tune_stage = 0;   // method 0 over several waves: 1 lattice in HBM, 2 staged in LDS as bytes, 3 as bits (0: automatic)
 thread_local gasp_run_stats_t t_stats{};
 
 // Thresholds in kernel order j = 5*s01 + (number of up neighbours).  With
@@ -245,14 +245,20 @@ static int layout_stride(int method, int N) {
 // share each entry, and a 20-sweep launch there takes 3.98 ms against 4.33
 // with 8192 entries; the batched replay, whose other groups then find less
 // LDS on those CUs, still went from 5.5-5.8 to 5.4-5.5 ms.
-// `stage`: the replica's bytes are staged in LDS below the tables (their
-// budget shrinks by the lattice; 0 entries when the lattice leaves no room).
-static int hbm_table_entries(int N, int R, bool stage) {
+// `stage` (the kernel's STAGE): 1 or 2, the replica's spins are staged in LDS
+// below the tables as bytes or bits (their budget shrinks by the lattice; 0
+// entries when the lattice leaves no room); 0, they stay in HBM.
+static size_t stage_bytes(int stage, int N) {
+    if (stage == 1) return (size_t)layout_stride(0, N);
+    if (stage == 2) return (((size_t)N + 127) / 128) * 16;  // N bits in 16-byte units
+    return 0;
+}
+static int hbm_table_entries(int N, int R, int stage) {
     DeviceInfo d = device_info();
     const int ncu = d.ok ? d.ncu : 256;
     const size_t lds = d.ok ? d.lds_block : 65536;
     const int per_cu = std::min(4, std::max(1, (R + ncu - 1) / std::max(ncu, 1)));
-    const size_t fixed = (size_t)kLdsLattice + (stage ? (size_t)layout_stride(0, N) : 0);
+    const size_t fixed = (size_t)kLdsLattice + stage_bytes(stage, N);
     const size_t per_wg = lds / (size_t)per_cu;  // at least 16 KB (at most four per CU)
     if (per_wg < fixed + 16) return 0;            // only a staged lattice can leave no room
     const size_t budget = (per_wg - fixed) / 8;
@@ -261,13 +267,26 @@ static int hbm_table_entries(int N, int R, bool stage) {
     return M;
 }
 
-// Method 0 over several waves stages the replica's bytes in LDS (the rounds
-// then wait on LDS instead of L2) where they fit beside tables of at least
-// min(N, 4096) entries for the workgroups a CU should hold.
-static bool hbm_stage(int N, int R) {
-    if (g_tune_stage == 1) return false;
-    const int M = hbm_table_entries(N, R, true);
-    return M > 0 && (g_tune_stage == 2 || M >= std::min(N, 4096));
+// Method 0 over several waves stages the replica's spins in LDS (the rounds
+// then wait on LDS instead of L2); returns the kernel's STAGE.  As bits a
+// workgroup's lattice takes an eighth of the LDS of bytes, for one more shift
+// and mask per spin read, and leaves the dependence tables more entries
+// (fewer spurious deferrals).  Automatic: bits where they leave the tables
+// more entries than bytes would (from L = 179 at one workgroup per CU, from
+// smaller L at several), else bytes where the tables keep min(N, 4096)
+// entries, else HBM.  20 sweeps of 256 replicas at L = 256 over eight waves
+// take 2.57 ms as bits (16384 entries) against 2.65 as bytes (8192; 2.79 as
+// bits with 8192); 5 sweeps at L = 512, 2.84 ms as bits against 9.55 in HBM;
+// 8 at L = 384, 2.41 against 6.69 in HBM and 8.72 as bytes beside 1024
+// entries.
+static int hbm_stage(int N, int R) {
+    if (g_tune_stage == 1) return 0;
+    if (g_tune_stage == 3) return hbm_table_entries(N, R, 2) > 0 ? 2 : 0;
+    const int M = hbm_table_entries(N, R, 1);
+    if (g_tune_stage == 2) return M > 0 ? 1 : 0;
+    const int M1 = hbm_table_entries(N, R, 2);
+    if (M1 > M) return 2;
+    return M > 0 && M >= std::min(N, 4096) ? 1 : 0;
 }
 
 // Method 2: `waves` is the word split (1, 2, 4, 8: 64, 32, 16, 8 replicas per
@@ -280,8 +299,8 @@ static size_t lds_bytes_for(int method, int N, int S, int waves, int R) {
         return (size_t)kLdsLattice + (size_t)rpw * stride * 4;
     }
     if (S > 64) {  // sweep_hbm_waves: the tables, after the staged lattice
-        const bool stage = hbm_stage(N, R);
-        return (size_t)kLdsLattice + (stage ? (size_t)stride : 0) + 8 * (size_t)hbm_table_entries(N, R, stage);
+        const int stage = hbm_stage(N, R);
+        return (size_t)kLdsLattice + stage_bytes(stage, N) + 8 * (size_t)hbm_table_entries(N, R, stage);
     }
     return (size_t)kLdsLattice;
 }
@@ -451,7 +470,7 @@ static SweepFn pick_marks(int S) {
 
 // variant: for the split-word layouts 0 (row/column filter) or the mark
 // plane's offset above the lattice (4096 .. 32768 bytes); for the byte
-// lattice over several waves 1 if it is staged in LDS.
+// lattice over several waves 1 if it is staged in LDS as bytes, 2 as bits.
 template <int LAT, int RNG, bool P2>
 static SweepFn pick_sweep(int S, int variant) {
     if constexpr (LAT == LAT_MULTIHALF || LAT == LAT_MULTI16 || LAT == LAT_MULTI8) {
@@ -479,15 +498,20 @@ static SweepFn pick_sweep(int S, int variant) {
     }
     if constexpr (LAT == LAT_HBM) {
         if (variant == 1) {
-            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, true>;
-            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, true>;
-            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, true>;
-            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, true>;
+            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, 1>;
+            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, 1>;
+            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, 1>;
+            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, 1>;
+        } else if (variant == 2) {
+            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, 2>;
+            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, 2>;
+            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, 2>;
+            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, 2>;
         } else {
-            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, false>;
-            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, false>;
-            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, false>;
-            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, false>;
+            if (S == 128) return sweep_hbm_waves<RNG, 2, P2, 0>;
+            if (S == 256) return sweep_hbm_waves<RNG, 4, P2, 0>;
+            if (S == 512) return sweep_hbm_waves<RNG, 8, P2, 0>;
+            if (S == 1024) return sweep_hbm_waves<RNG, 16, P2, 0>;
         }
     }
     return nullptr;
@@ -550,7 +574,7 @@ static int ilog2(int L) {
 struct LaunchShape {
     int S = 1, W = 1, stride = 0, rpw = 0, nblocks = 0, block = 0;
     int tbl = 1;         // sweep_hbm_waves: conflict-table entries
-    bool stage = false;  // sweep_hbm_waves: the lattice staged in LDS
+    int stage = 0;       // sweep_hbm_waves: STAGE (0 HBM, 1 LDS bytes, 2 LDS bits)
     int ncu = 256;
     size_t lds = 0, lat_bytes = 0;
     bool sliced = false;  // method 3: sweep_sliced, S = waves per workgroup, W = word split (1: 64, 2: 32)
@@ -582,7 +606,7 @@ static int launch_shape(int method, int L, int R, LaunchShape *sh) {
     DeviceInfo di = device_info();
     if (di.ok) sh->ncu = di.ncu;
     sh->lds = lds_bytes_for(method, N, sh->S, sh->W, R);
-    sh->stage = method == 0 && sh->S > 64 && hbm_stage(N, R);
+    sh->stage = (method == 0 && sh->S > 64) ? hbm_stage(N, R) : 0;
     sh->tbl = (method == 0 && sh->S > 64) ? hbm_table_entries(N, R, sh->stage) : 1;
     if (method == 2) {
         const int groups = (R + 63) / 64;
@@ -663,7 +687,7 @@ struct gasp_engine {
     int freeze = 0;          // skip replicas whose fate is decided
     int S0 = 1;              // slots per replica chosen at creation
     int moff = 0;            // mark plane offset above the lattice (0: row/column filter)
-    bool stage = false;      // method 0 over several waves: the lattice staged in LDS
+    int stage = 0;           // method 0 over several waves: 1 lattice staged in LDS as bytes, 2 as bits
     bool adapt = false;      // raise the slots as replicas are decided (see engine_adapt_slots)
     int decided = 0;         // decided replicas at the last counts read (or set_fates)
     int ncu = 256;
@@ -763,7 +787,7 @@ static int engine_new(int L, int R, int method, gasp_rng_state *d_state, const u
     e->block = sh.block;
     e->nblocks = sh.nblocks;
     e->lat_bytes = sh.lat_bytes;
-    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L), e->stage ? 1 : e->moff);
+    SweepFn fn = sweep_fn(method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(L), e->stage ? e->stage : e->moff);
     if (!fn) {
         engine_free(e);
         return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", e->S);
@@ -880,7 +904,7 @@ static int launch_sweep(gasp_engine *e, uint64_t n_attempts, const uint32_t *dra
     a.dn_thr = dn;
     a.up_thr = up;
     SweepFn fn = sweep_fn(e->method, e->rng.kind == KIND_MT ? 1 : 0, e->S, e->W, is_pow2(e->L),
-                          e->stage ? 1 : e->moff);
+                          e->stage ? e->stage : e->moff);
     if (check != CHECK_NONE) {
         hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, e->stream, e->d_counts, 4);
         HIP_TRY(hipGetLastError());
@@ -1284,7 +1308,7 @@ extern "C" int gasp_engine_get_conflict_filter(gasp_engine *e, int *filter) {
 
 extern "C" int gasp_engine_get_lattice_staging(gasp_engine *e, int *staging) {
     if (!e || !staging) return set_error(GASP_ERR_ARG, "gasp_engine_get_lattice_staging: NULL argument");
-    *staging = e->stage ? 2 : 1;
+    *staging = e->stage + 1;
     return GASP_OK;
 }
 
@@ -1350,7 +1374,7 @@ static int sweep_grids_run(int L, int R, int method, gasp_rng_state *d_state, do
     else
         hipLaunchKernelGGL(pack_hbm, grid, blk, 0, st, d_grids, (unsigned char *)lat, R, N, sh.stride, 1);
     HIP_TRY(hipGetLastError());
-    SweepFn fn = sweep_fn(method, 0, sh.S, sh.W, is_pow2(L), sh.stage ? 1 : 0);
+    SweepFn fn = sweep_fn(method, 0, sh.S, sh.W, is_pow2(L), sh.stage);
     if (!fn) return set_error(GASP_ERR_ARG, "Error: no kernel for S=%d", sh.S);
     if (sh.lds > 65536)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));

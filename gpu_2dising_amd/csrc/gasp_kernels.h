@@ -35,7 +35,8 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 constexpr uint32_t kPhiloxDomain = 0x47415350u;  // "GASP"
 
-enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3, LAT_MULTI16 = 4, LAT_MULTI8 = 5, LAT_LDS8 = 6 };
+enum { LAT_HBM = 0, LAT_BITS = 1, LAT_MULTISPIN = 2, LAT_MULTIHALF = 3, LAT_MULTI16 = 4, LAT_MULTI8 = 5, LAT_LDS8 = 6,
+       LAT_LDS1 = 7 };
 enum { RNG_PHILOX = 0, RNG_DRAWS = 1 };
 enum { CHECK_NONE = 0, CHECK_COUNT = 1, CHECK_FATES = 2 };
 
@@ -240,6 +241,10 @@ struct Geometry {
 // Layout 6 (layout 0 staged in LDS): one replica's bytes at LDS address
 //   kLdsLattice + site, for the multi-wave sweep of one replica per workgroup;
 //   its barriers order the flips (plain byte stores) before the next reads.
+// Layout 7 (layout 0 staged in LDS as bits): one replica's spins as bit
+//   (site & 31) of the dword at LDS address kLdsLattice + 4 (site >> 5), an
+//   eighth of layout 6's LDS; a flip is an atomic XOR (a round's flips are at
+//   distinct sites, but several may share a dword).
 template <int LAT>
 struct Lattice {
     unsigned char *base;  // LDS (1/2) or global (0) base of this lane's replica
@@ -263,6 +268,9 @@ struct Lattice {
             return (base[site >> 3] >> (site & 7)) & 1;
         } else if constexpr (LAT == LAT_LDS8) {
             return (int)*(const lds_u8 *)(uintptr_t)(kLdsLattice + (unsigned)site);
+        } else if constexpr (LAT == LAT_LDS1) {
+            return (int)__builtin_amdgcn_ubfe(lds_load32(kLdsLattice + (((unsigned)site >> 5) << 2)),
+                                              (uint32_t)site & 31u, 1u);
         } else {
             return (int)__builtin_amdgcn_raw_buffer_load_b8(rsrc, off + (unsigned)site, 0, 0);
         }
@@ -279,6 +287,8 @@ struct Lattice {
             atomicXor((unsigned *)base + (site >> 5), 1u << (site & 31));
         } else if constexpr (LAT == LAT_LDS8) {
             *(lds_u8 *)(uintptr_t)(kLdsLattice + (unsigned)site) = (unsigned char)(old ^ 1);
+        } else if constexpr (LAT == LAT_LDS1) {
+            lds_xor32(kLdsLattice + (((unsigned)site >> 5) << 2), 1u << ((unsigned)site & 31u));
         } else {
             __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(old ^ 1), rsrc, off + (unsigned)site, 0, 0);
         }
@@ -989,21 +999,26 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
 // table during the next round.  Two barriers per round: after the marks,
 // and after the flips (which the next round's reads must see) and the counts
 // of deferred attempts per wave.
-// STAGE: the replica's bytes are staged in LDS (layout 6) at kLdsLattice for
-// the launch, and the tables follow them; otherwise they stay in HBM.
-template <int RNG, int W, bool POW2, bool STAGE>
+// STAGE: 0, the replica's bytes stay in HBM; 1, they are staged in LDS for
+// the launch as bytes (layout 6), 2, as bits (layout 7), at kLdsLattice, and
+// the tables follow them.
+template <int RNG, int W, bool POW2, int STAGE>
 __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     static_assert(W == 2 || W == 4 || W == 8 || W == 16, "2, 4, 8 or 16 waves per replica");
+    static_assert(STAGE >= 0 && STAGE <= 2, "HBM, LDS bytes or LDS bits");
     constexpr int S = 64 * W;
-    constexpr int LAT = STAGE ? LAT_LDS8 : LAT_HBM;
+    constexpr int LAT = STAGE == 1 ? LAT_LDS8 : STAGE == 2 ? LAT_LDS1 : LAT_HBM;
     constexpr unsigned kNone = 0xffffffffu;
     extern __shared__ __align__(16) unsigned char smem[];
     uint64_t *thr_lds = (uint64_t *)(smem + kLdsThr);
     unsigned *cnt_lds = (unsigned *)(smem + kLdsCnt);  // [0]: up spins; [16 + 16 par + wave]: deferred attempts
     const unsigned hm = a.tbl_mask;
     const unsigned tsz = 4u * (hm + 1u);  // bytes per table; table p at tbase + p tsz
-    // a.stride (bytes per replica) is a multiple of 16
-    const unsigned tbase = (unsigned)kLdsLattice + (STAGE ? (unsigned)a.stride : 0u);
+    // a.stride (bytes per replica) is a multiple of 16; layout 7 takes N bits
+    // rounded up to 16 bytes
+    const unsigned nbw = ((unsigned)a.N + 31u) >> 5;  // layout 7: dwords
+    const unsigned tbase = (unsigned)kLdsLattice +
+                           (STAGE == 1 ? (unsigned)a.stride : STAGE == 2 ? ((nbw + 3u) >> 2) << 4 : 0u);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -1025,10 +1040,24 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     lat.rl = 0;
     lat.rsrc = __builtin_amdgcn_make_buffer_rsrc(lat.base, (short)0, a.stride, kBufferWord3);
     lat.off = 0u;
-    if constexpr (STAGE) {
+    if constexpr (STAGE == 1) {
         const uint4 *src = (const uint4 *)lat.base;
         uint4 *dst = (uint4 *)(smem + kLdsLattice);
         for (int i = tid; i < a.stride / 16; i += S) dst[i] = src[i];
+    } else if constexpr (STAGE == 2) {
+        // bit k of dword i from byte 32 i + k (0 or 1); loads past the
+        // replica's stride read 0, and the bits of sites from N on are cleared
+        for (unsigned i = (unsigned)tid; i < nbw; i += S) {
+            unsigned v = 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(lat.rsrc, 32u * i + 4u * (unsigned)k, 0, 0);
+                v |= ((w & 1u) | ((w >> 7) & 2u) | ((w >> 14) & 4u) | ((w >> 21) & 8u)) << (4 * k);
+            }
+            const unsigned rest = (unsigned)a.N - 32u * i;
+            if (rest < 32u) v &= (1u << rest) - 1u;
+            *lds_ptr32(kLdsLattice + 4u * i) = v;
+        }
     }
     __syncthreads();
 
@@ -1119,7 +1148,19 @@ __global__ void __launch_bounds__(64 * W) sweep_hbm_waves(SweepArgs a) {
     // ---------------------------------------------------------- epilogue
     __syncthreads();
     unsigned c = 0;
-    if constexpr (STAGE) {
+    if constexpr (STAGE == 2) {
+        // the bits back to bytes (sites from N on, within the stride, get 0)
+        for (unsigned i = (unsigned)tid; i < nbw; i += S) {
+            const unsigned v = lds_load32(kLdsLattice + 4u * i);
+            c += __popc(v);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const unsigned q = v >> (4 * k);
+                const unsigned w = (q & 1u) | ((q & 2u) << 7) | ((q & 4u) << 14) | ((q & 8u) << 21);
+                __builtin_amdgcn_raw_buffer_store_b32(w, lat.rsrc, 32u * i + 4u * (unsigned)k, 0, 0);
+            }
+        }
+    } else if constexpr (STAGE == 1) {
         const uint4 *src = (const uint4 *)(smem + kLdsLattice);
         uint4 *dst = (uint4 *)lat.base;
         for (int i = tid; i < a.stride / 16; i += S) {

@@ -241,9 +241,12 @@ void gasp_set_conflict_filter(int filter);
 /* Method 0 with 128 to 1024 slots (one replica over 2 to 16 waves), for
  * engines created afterwards: 1 = the byte lattice stays in HBM (read through
  * L2 every round), 2 = it is staged in LDS for each launch where it fits beside
- * the dependence tables (L up to about 330 at one workgroup per CU), 0 (or any
- * other value) = automatic: staged where the tables keep min(N, 4096) entries.
- * Trajectories do not depend on it.  No counterpart in the reference (a tuning
+ * the dependence tables (L up to about 330 at one workgroup per CU), 3 = it is
+ * staged in LDS as bits (an eighth of the LDS, one more shift and mask per spin
+ * read), 0 (or any other value) = automatic: as bits where that leaves the
+ * dependence tables more entries than bytes would (large L), else as bytes
+ * where the tables keep min(N, 4096) entries, else in HBM.  Trajectories do
+ * not depend on it.  No counterpart in the reference (a tuning
  * knob of this engine, like gasp_set_tuning). */
 void gasp_set_lattice_staging(int staging);
 

@@ -191,6 +191,36 @@ def test_conflict_filter_choice(gpu, engine_lib):
         G.set_conflict_filter(0)
 
 
+def test_lattice_staging_choice(gpu, engine_lib):
+    """Automatic staging of method 0 over several waves on a 160 KB-LDS
+    device: as bits where that leaves the dependence tables more entries than
+    bytes (from L = 179 at one workgroup per CU; at L = 128 once four
+    workgroups share a CU; L = 512, where bytes do not fit), else as bytes;
+    1, 2 and 3 force HBM, bytes and bits."""
+    G = engine_lib
+
+    def stg(L, R, S=512):
+        G.set_tuning(S, S // 64)
+        rng = G.RngState(R, 5)
+        try:
+            with G.Engine(L, R, 0, rng, BETA, H) as e:
+                return e.lattice_staging()
+        finally:
+            G.set_tuning(0, 0)
+            rng.free()
+
+    assert (stg(178, 64), stg(179, 64)) == (2, 3)
+    assert stg(64, 64) == 2
+    assert (stg(128, 256, S=256), stg(128, 1024, S=256)) == (2, 3)
+    assert stg(512, 64) == 3
+    try:
+        for forced, (L, R) in ((1, (256, 64)), (2, (256, 64)), (3, (64, 64))):
+            G.set_lattice_staging(forced)
+            assert stg(L, R) == forced
+    finally:
+        G.set_lattice_staging(0)
+
+
 def test_set_attempt_rewinds_streams(gpu, engine_lib, oracle):
     """gasp_engine_set_attempt: after sweeps, rewinding to attempt 0 and
     reloading the start grids replays the same trajectory; moving to attempt
@@ -223,14 +253,15 @@ def test_set_attempt_rewinds_streams(gpu, engine_lib, oracle):
         mt.free()
 
 
-@pytest.mark.parametrize("staging", [1, 2])
+@pytest.mark.parametrize("staging", [1, 2, 3])
 @pytest.mark.parametrize("S", [128, 256, 512, 1024])
 def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S, staging):
     """A replica spread over 2, 4, 8 or 16 waves at the batched run's L = 256
     shape (R = 6, so the conflict table has fewer entries than sites and
     sites share entries), with the byte lattice in HBM (staging 1) or staged
-    in LDS (2), from all -1 at the reference's (beta, h) and from a random
-    start at beta = 0.3 (about half the attempts accepted)."""
+    in LDS as bytes (2) or bits (3), from all -1 at the reference's (beta, h)
+    and from a random start at beta = 0.3 (about half the attempts
+    accepted)."""
     G, O = engine_lib, oracle
     L, R, seed = 256, 6, 4242
     G.set_lattice_staging(staging)
@@ -248,6 +279,38 @@ def test_hbm_waves_at_L256(gpu, engine_lib, oracle, S, staging):
             ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
             assert np.array_equal(got, ref)
             assert np.array_equal(mag, (ref.reshape(R, -1).sum(1) / np.float32(L * L)).astype(np.float32))
+    finally:
+        G.set_lattice_staging(0)
+
+
+@pytest.mark.parametrize("mt", [False, True])
+@pytest.mark.parametrize("L,S", [(17, 128), (45, 512), (64, 256)])
+def test_hbm_waves_bits_staged(gpu, engine_lib, oracle, L, S, mt):
+    """The byte lattice staged in LDS as bits (staging 3) on lattices whose
+    site count is not a multiple of 32 (the last dword is partial: 289 and
+    2025 sites) and on one that is, with both RNG streams, from a random start
+    at beta = 0.3."""
+    G, O = engine_lib, oracle
+    R, seed, beta = 7, 31337, 0.3
+    G.set_lattice_staging(3)
+    try:
+        rng = G.RngState(R, 1)
+        G.set_tuning(S, S // 64)
+        with G.Engine(L, R, 0, rng, beta, H) as e:
+            assert e.lattice_staging() == 3
+        G.set_tuning(0, 0)
+        rng.free()
+        print(f"[bits staged] L={L} S={S} mt={mt}", flush=True)
+        g0 = _start("random", R, L, seed)
+        got, mag = _run_engine(G, L, R, 0, S, g0, seed, mt, beta, H, [2, 1], W=S // 64)
+        if mt:
+            ref = O.sweep_mt_per_replica(L, g0.copy(), seed, 3, beta, H)
+        else:
+            ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
+        ref_m = np.array([O.lib().orc_magnetisation_gpu(L, ref[r].reshape(-1)) for r in range(R)],
+                         dtype=np.float32)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(mag, ref_m)
     finally:
         G.set_lattice_staging(0)
 
