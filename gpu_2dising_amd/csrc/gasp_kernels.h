@@ -779,6 +779,17 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                 if (m != 0ull) {
                     const bool accA = __builtin_amdgcn_inverse_ballot_w64(mA);
                     const bool accB = __builtin_amdgcn_inverse_ballot_w64(mB);
+                    // Only the lanes after their group's first lane with an
+                    // accepted attempt can follow an accepted attempt of
+                    // another lane, so only they read their stencils' marks
+                    // (at config 2 about half the lanes skip the ten reads).
+                    // An accepted attempt among the others still reports a
+                    // mark at its own site through the old mark its OR
+                    // returned: of two ORs at one site in one instruction only
+                    // the second sees the first, and that may be either lane.
+                    uint64_t rd = group_after<S>(m);
+                    asm volatile("" : "+s"(rd));
+                    const bool reads = __builtin_amdgcn_inverse_ballot_w64(rd);
                     bool hA, hB;
                     if constexpr (LAT == LAT_MULTIHALF) {
                         // whole dwords: one test of the replica's bit at the end
@@ -788,15 +799,24 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                         if (accA) ownA = lds_or_rtn32(M + (unsigned)A.st.c, bit);
                         if (accB) ownB = lds_or_rtn32(M + (unsigned)B.st.c, bit);
                         asm volatile("" ::: "memory");
-                        hA = (half_marks<MOFF>(A.st, accA, ownA) & bit) != 0u;
-                        hB = (half_marks<MOFF>(B.st, accB, ownB) & bit) != 0u;
+                        unsigned gA = ownA, gB = ownB;  // 0 unless accepted
+                        if (reads) {
+                            gA = half_marks<MOFF>(A.st, accA, ownA);
+                            gB = half_marks<MOFF>(B.st, accB, ownB);
+                        }
+                        hA = (gA & bit) != 0u;
+                        hB = (gB & bit) != 0u;
                     } else {
                         bool ownA = false, ownB = false;
                         if (accA) ownA = mark_site<LAT, MOFF>(A.st.c, rl);
                         if (accB) ownB = mark_site<LAT, MOFF>(B.st.c, rl);
                         asm volatile("" ::: "memory");
-                        hA = stencil_hit<LAT, MOFF>(A.st, rl, accA, ownA);
-                        hB = stencil_hit<LAT, MOFF>(B.st, rl, accB, ownB);
+                        hA = ownA;
+                        hB = ownB;
+                        if (reads) {
+                            hA = stencil_hit<LAT, MOFF>(A.st, rl, accA, ownA);
+                            hB = stencil_hit<LAT, MOFF>(B.st, rl, accB, ownB);
+                        }
                     }
                     const uint64_t cand = (eA & wave_mask(hA)) | (eB & wave_mask(hB));
                     m = cand ? (m & group_span<S>(cand)) : 0ull;
