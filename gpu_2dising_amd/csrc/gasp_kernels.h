@@ -515,16 +515,19 @@ __device__ __forceinline__ void finish_replica(const SweepArgs &a, bool own, int
 // (an entry of 32, 16 or 8 bits per site, bit rl = replica): a second plane of
 // the same shape, MOFF bytes above the lattice, all zero between rounds.  Each
 // accepted attempt ORs its replica's bit into its site's mark (returning the
-// old mark), then every attempt reads the marks of its five stencil sites, at
-// the lattice addresses plus MOFF.  An attempt whose stencil holds the site of
-// an accepted attempt of its replica from the same round sees that bit: at a
-// neighbour through the neighbour's mark; at its own site through the site's
-// mark if it did not accept, and if it did through the old mark returned to
-// one of the two ORs (a wave's lanes and its successive instructions reach
-// LDS in order).  Those attempts are the candidates; the exact loop sorts out
-// which accepted attempt came first.  Unlike the row and column masks, whose
-// unions admit a growing share of attempts as a group's accepted attempts
-// multiply, it admits none without an accepted attempt in the stencil.
+// old mark), then every attempt after an accepted attempt of its group reads
+// the marks of its five stencil sites (the earlier ones cannot follow an
+// accepted attempt), at the lattice addresses plus MOFF.  An attempt whose
+// stencil holds the site of an accepted attempt of its replica from the same
+// round sees that bit: at a neighbour through the neighbour's mark; at its own
+// site through the site's mark if it did not accept, and if it did through the
+// old mark returned to one of the two ORs (a wave's lanes and its successive
+// instructions reach LDS in order; an accepted attempt that skips the reads
+// still has that old mark).  Those attempts are the candidates; the exact loop
+// sorts out which accepted attempt came first.  Unlike the row and column
+// masks, whose unions admit a growing share of attempts as a group's accepted
+// attempts multiply, it admits none without an accepted attempt in the
+// stencil.
 // The accepted attempts clear their bits at the end of the round.
 __device__ __forceinline__ unsigned lds_or_rtn32(unsigned addr, unsigned v) {
     return __hip_atomic_fetch_or(lds_ptr32(addr), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
