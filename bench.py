@@ -288,7 +288,8 @@ def batched_leg(G, replays: int = 10, chk: int = 20, seed: int = 2026) -> dict:
     seq_ms = (time.perf_counter() - t0) * 1e3 / replays
     groups = [{"L": g.L, "replicas": g.R, "method": g.method, "slots": g.slots, "waves": g.waves,
                "sweeps_per_launch": g.launch_sweeps, "probe_rate": g.probe_rate, "beta": g.beta, "h": g.h,
-               "itask": g.itask} for g in b.groups]
+               "itask": g.itask, "conflict_filter": g.engine.conflict_filter(),
+               "lattice_staging": g.engine.lattice_staging()} for g in b.groups]
     order = [b.groups[i].L for i in b.order]
     tuned_ms, refine_log = b.tuned_ms, b.refine_log
     b.close()

@@ -31,6 +31,7 @@ EXPORTS = (
     "gasp_last_error", "gasp_error_string", "gasp_clear_error", "gasp_init_rand_mt", "gasp_free",
     "gasp_set_tuning", "gasp_get_tuning", "gasp_get_run_stats", "gasp_set_conflict_filter",
     "gasp_engine_get_conflict_filter", "gasp_set_lattice_staging", "gasp_engine_get_lattice_staging",
+    "gasp_set_dependence_tables",
     "gasp_engine_set_attempt",
     "gasp_engine_create", "gasp_engine_destroy", "gasp_engine_set_grids", "gasp_engine_get_grids",
     "gasp_engine_gather_grids", "gasp_engine_get_gridstates",
@@ -132,6 +133,7 @@ def lib() -> C.CDLL:
     L.gasp_set_tuning.argtypes = [C.c_int, C.c_int]
     L.gasp_set_conflict_filter.argtypes = [C.c_int]
     L.gasp_set_lattice_staging.argtypes = [C.c_int]
+    L.gasp_set_dependence_tables.argtypes = [C.c_int]
     L.gasp_get_tuning.argtypes = [C.c_int, C.c_int, C.c_int, ip, ip]
     L.gasp_get_run_stats.argtypes = [C.POINTER(RunStats)]
     L.gasp_engine_create.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double,
@@ -485,6 +487,12 @@ def bootstrap_libc(fates: np.ndarray, m: int, seed: int = 1, nthreads: int = 0) 
 
 def set_tuning(slots: int = 0, waves: int = 0) -> None:
     lib().gasp_set_tuning(slots, waves)
+
+
+def set_dependence_tables(max_entries: int = 0) -> None:
+    """Method 0 over several waves: at most `max_entries` entries per LDS
+    dependence table for engines created afterwards (0 = up to 16384)."""
+    lib().gasp_set_dependence_tables(max_entries)
 
 
 def set_lattice_staging(staging: int = 0) -> None:

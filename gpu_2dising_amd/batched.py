@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import (CHECK_COUNT, CHECK_FATES, Engine, Graph, RngState, autotune, set_conflict_filter,
-               set_lattice_staging, set_tuning)
+               set_dependence_tables, set_lattice_staging, set_tuning)
 
 # L, replicas, gpu_method: four lattice sizes on the LDS multispin path and
 # one on the HBM byte lattice
@@ -59,50 +59,51 @@ class Batch:
     def __init__(self, groups=CONFIG4_GROUPS, chk: int = 20, seed: int = 2026, tune: bool = True,
                  probe_sweeps: int = 4, dn: float = -0.9, up: float = 0.9, shapes=None,
                  conflict_filter=None, refine: bool = False, refine_replays: int = 10,
-                 lattice_staging: int = 2):
+                 lattice_staging: int = 3, dependence_tables: int = 8192):
         """`shapes`: optional (slots, waves, sweeps per launch) per group, used
         instead of the tuner's choice (for comparing builds at fixed shapes).
         `conflict_filter`: the method-2 filter of the groups' engines (as
-        set_conflict_filter takes it), one for all or one per group; the
-        process setting returns to automatic afterwards.  By default the
-        shapes are tuned with the row/column masks, and only half-word groups
-        whose mark plane is at most 4 KB (L <= 32) then take the plane: it
-        doubles a workgroup's LDS, and in the graph fewer workgroups of the
-        larger groups fit beside the large-L group's.  At the batched
-        configuration's shapes replays took 5.16-5.24 ms with the plane for
-        L = 32 only, 5.15-5.50 with none, 5.7-6.0 with L = 48 too, 6.5-7.1
-        with L = 96 and 6.1-6.4 with every method-2 group on it, although
-        each group alone ran faster with it.
+        set_conflict_filter takes it), one for all or one per group, for the
+        tuner's probes and the engines; the process setting returns to
+        automatic afterwards.  By default the mark plane for every method-2
+        group (an engine takes the row/column masks where the plane does not
+        fit or apply).
+        `lattice_staging`, `dependence_tables`: the staging of multi-wave
+        method-0 groups and the most entries of their dependence tables (as
+        set_lattice_staging and set_dependence_tables take them); by default
+        bits in LDS beside 8192-entry tables (74.6 KB per workgroup at
+        L = 256).
+        Together: the mark plane doubles a method-2 workgroup's LDS, and in
+        the graph the groups share the CUs' LDS with the L = 256 group.  At
+        the batched configuration's refined shapes, with the L = 256 group on
+        bytes beside 8192-entry tables (129 KB per workgroup), replays took
+        4.72-4.75 ms with the plane for L = 32 only and 6.6-7.2 ms with it on
+        L = 48, 64 or 96 as well; with that group on bits beside 8192 entries,
+        4.59-4.61 ms with the plane on every method-2 group, 4.70 on L = 32
+        and 96, 4.90 on L = 32 only; on bits beside 16384 entries (the
+        engine's own choice, 140 KB) 5.57 ms.
         `refine`: after the per-group tuning, a coordinate search over the
-        groups' shapes on the captured graph itself (see `_refine`).
-        `lattice_staging`: the staging of multi-wave method-0 groups (as
-        set_lattice_staging takes it) for the tuner's probes and the engines;
-        by default bytes in LDS.  Alone, L = 256 with 256 replicas runs 3%
-        faster staged as bits (the engine's automatic choice there: its
-        workgroup keeps 16384-entry dependence tables in 140 KB of LDS), but
-        the batched configuration's refined graph replayed in 5.57 ms that
-        way against 4.77 with bytes (8192 entries, 129 KB), and in 4.91 ms with
-        bits beside 8192 entries."""
-        auto = conflict_filter is None
-        filters = ([1] * len(groups) if auto else
+        groups' shapes on the captured graph itself (see `_refine`)."""
+        filters = ([2 if m == 2 else 1 for (_, _, m) in groups] if conflict_filter is None else
                    list(conflict_filter) if isinstance(conflict_filter, (list, tuple))
                    else [conflict_filter] * len(groups))
         if len(filters) != len(groups):
             raise ValueError("conflict_filter: one value, or one per group")
-        self.auto_filter = auto
-        self.staging = lattice_staging
+        self.staging, self.tables = lattice_staging, dependence_tables
         self.refine_log, self.tuned_ms, self.refined_ms = [], None, None
         set_lattice_staging(lattice_staging)
+        set_dependence_tables(dependence_tables)
         try:
-            self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto)
+            self._build(groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters)
             if refine and shapes is None:
                 self._refine(refine_replays)
         finally:
             set_conflict_filter(0)
             set_lattice_staging(0)
+            set_dependence_tables(0)
             set_tuning(0, 0)
 
-    def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters, auto):
+    def _build(self, groups, chk, seed, tune, probe_sweeps, dn, up, shapes, filters):
         rs = np.random.default_rng(seed)
         self.chk, self.dn, self.up = chk, dn, up
         self.groups: list[Group] = []
@@ -136,18 +137,14 @@ class Batch:
         self.order = order
         self._capture()
 
-    def _filter(self, k: int) -> int:
-        g = self.groups[k]
-        if self.auto_filter:  # the mark plane for half-word groups whose plane is at most 4 KB
-            return 0 if g.method == 2 and g.waves == 2 and 4 * g.L * g.L <= 4096 else 1
-        return self.filters[k]
 
     def _make_engine(self, k: int, fixed: bool) -> None:
         """Group k's RNG streams (from attempt 0) and engine, at its shape if
         `fixed` (else the tuner's), loaded with its start state."""
         g = self.groups[k]
-        set_conflict_filter(self._filter(k))
+        set_conflict_filter(self.filters[k])
         set_lattice_staging(self.staging)
+        set_dependence_tables(self.tables)
         g.rng = RngState(g.R, g.seed)
         if fixed:
             set_tuning(g.slots, g.waves)

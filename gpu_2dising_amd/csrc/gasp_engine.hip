@@ -99,6 +99,7 @@ int g_tune_slots = 0;
 int g_tune_waves = 0;
 int g_tune_filter = 0;  // method 2 on half words: 1 row/column masks, 2 mark plane (0: automatic)
 int g_tune_stage = 0;   // method 0 over several waves: 1 lattice in HBM, 2 staged in LDS as bytes, 3 as bits (0: automatic)
+int g_tune_tables = 0;  // method 0 over several waves: most entries per dependence table (0: 16384)
 thread_local gasp_run_stats_t t_stats{};
 
 // Thresholds in kernel order j = 5*s01 + (number of up neighbours).  With
@@ -220,6 +221,7 @@ extern "C" void gasp_set_tuning(int slots_per_replica, int waves_per_group) {
 
 extern "C" void gasp_set_conflict_filter(int filter) { g_tune_filter = filter; }
 extern "C" void gasp_set_lattice_staging(int staging) { g_tune_stage = staging; }
+extern "C" void gasp_set_dependence_tables(int max_entries) { g_tune_tables = max_entries; }
 
 static int pow2_ceil(long long x) {
     int p = 1;
@@ -240,7 +242,8 @@ static int layout_stride(int method, int N) {
 // Entries of each of sweep_hbm_waves' two LDS dependence tables (a power of
 // two): one per site, up to what the LDS of a CU holds for the workgroups it
 // should run at once (R workgroups over the CUs, at most four per CU), and at
-// most 16384 (both tables 128 KB).  Fewer entries than sites only add
+// most 16384 (both tables 128 KB) or gasp_set_dependence_tables' limit.
+// Fewer entries than sites only add
 // deferrals (sites share entries): at the batched L = 256 shape four sites
 // share each entry, and a 20-sweep launch there takes 3.98 ms against 4.33
 // with 8192 entries; the batched replay, whose other groups then find less
@@ -263,7 +266,8 @@ static int hbm_table_entries(int N, int R, int stage) {
     if (per_wg < fixed + 16) return 0;            // only a staged lattice can leave no room
     const size_t budget = (per_wg - fixed) / 8;
     int M = 1;
-    while (M < N && M < 16384 && (size_t)M * 2 <= budget) M *= 2;
+    const int cap = g_tune_tables > 0 ? std::min(g_tune_tables, 16384) : 16384;
+    while (M < N && M * 2 <= cap && (size_t)M * 2 <= budget) M *= 2;
     return M;
 }
 

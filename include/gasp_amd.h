@@ -249,6 +249,14 @@ void gasp_set_conflict_filter(int filter);
  * not depend on it.  No counterpart in the reference (a tuning
  * knob of this engine, like gasp_set_tuning). */
 void gasp_set_lattice_staging(int staging);
+/* Method 0 with 128 to 1024 slots, for engines created afterwards: at most
+ * max_entries entries (the largest power of two not above it) in each of the
+ * two LDS dependence tables of a workgroup, which otherwise take up to 16384
+ * where the LDS of a CU allows (0 or less = that default).  Fewer entries leave
+ * LDS to other kernels on the same CUs (a batched graph) at the price of more
+ * deferred attempts; trajectories do not depend on it.  No counterpart in the
+ * reference (a tuning knob of this engine, like gasp_set_tuning). */
+void gasp_set_dependence_tables(int max_entries);
 
 /* Statistics of the most recent mc_driver_gpu / gasp_engine run on this
  * thread: sweep-kernel time (HIP events), attempts performed per replica,

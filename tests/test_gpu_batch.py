@@ -133,8 +133,9 @@ def test_graph_after_direct_sweeps(gpu, engine_lib, oracle):
 
 def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
     """The batched configuration at its real shapes: L = 32, 48, 64, 96 (LDS
-    multispin) with 8192, 4096, 4096, 2048 replicas and L = 256 (method 0, its
-    byte lattice staged in LDS) with 256, beta and h drawn per group, itask 0 and 1 alternating, launch shapes and
+    multispin, the mark plane) with 8192, 4096, 4096, 2048 replicas and L = 256
+    (method 0, its lattice staged in LDS as bits beside 8192-entry dependence
+    tables) with 256, beta and h drawn per group, itask 0 and 1 alternating, launch shapes and
     sweeps per launch from the autotuner, three replays of 20 sweeps.  For 12
     replicas of every group (first, last and random ones) the oracle replays
     the same Philox streams: grids, magnetisations and fates must be equal; the
@@ -144,12 +145,15 @@ def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
     G = engine_lib
     print("[batched] five groups at full shape, autotuned", flush=True)
     b = Batch(chk=20, seed=2026)
-    # the mark plane for the half-word groups with a plane of at most 4 KB (L = 32), the
-    # row/column masks elsewhere; the L = 256 lattice is staged in LDS
-    want = [2 if g.method == 2 and g.waves == 2 and 4 * g.L * g.L <= 4096 else 1 for g in b.groups]
+    # the mark plane for every method-2 group whose shape takes it (split words, 2 to 32
+    # slots, quarter and eighth words from 8, a plane of at most 32 KB); the L = 256
+    # lattice is staged in LDS as bits
+    def plane(g):
+        return (g.method == 2 and g.waves in (2, 4, 8) and 2 <= g.slots <= 32 and
+                (g.waves == 2 or g.slots >= 8) and (8 // g.waves) * g.L * g.L <= 32768)
+    want = [2 if plane(g) else 1 for g in b.groups]
     assert [g.engine.conflict_filter() for g in b.groups] == want
-    assert want[0] == 2 or b.groups[0].waves != 2
-    assert [g.engine.lattice_staging() for g in b.groups if g.method == 0] == [2]
+    assert [g.engine.lattice_staging() for g in b.groups if g.method == 0] == [3]
     replays = 3
     b.graph.launch(replays)
     rs = np.random.default_rng(5)

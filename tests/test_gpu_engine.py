@@ -219,6 +219,37 @@ def test_lattice_staging_choice(gpu, engine_lib):
             assert stg(L, R) == forced
     finally:
         G.set_lattice_staging(0)
+    # with at most 8192 entries per table, bits no longer leave more entries at L = 179
+    try:
+        G.set_dependence_tables(8192)
+        assert stg(179, 64) == 2
+        assert stg(512, 64) == 3
+    finally:
+        G.set_dependence_tables(0)
+
+
+@pytest.mark.parametrize("mt", [False, True])
+def test_dependence_table_limit_bit_exact(gpu, engine_lib, oracle, mt):
+    """gasp_set_dependence_tables: 64 entries per table for 2025 sites (about
+    32 sites share each entry, so most deferrals are spurious) on the
+    bits-staged lattice over four waves; the trajectories stay those of the
+    sequential chain."""
+    G, O = engine_lib, oracle
+    L, R, S, seed, beta = 45, 5, 256, 9091, 0.3
+    G.set_lattice_staging(3)
+    G.set_dependence_tables(64)
+    try:
+        print(f"[table limit] L={L} S={S} mt={mt}", flush=True)
+        g0 = _start("random", R, L, seed)
+        got, mag = _run_engine(G, L, R, 0, S, g0, seed, mt, beta, H, [2, 1], W=S // 64)
+        if mt:
+            ref = O.sweep_mt_per_replica(L, g0.copy(), seed, 3, beta, H)
+        else:
+            ref = O.sweep_philox(L, g0.copy(), seed, 0, 3 * L * L, beta, H)
+        assert np.array_equal(got, ref)
+    finally:
+        G.set_lattice_staging(0)
+        G.set_dependence_tables(0)
 
 
 def test_set_attempt_rewinds_streams(gpu, engine_lib, oracle):
