@@ -195,7 +195,7 @@ class Batch:
         slots = [s_ for s_ in (g.slots // 2, g.slots, 2 * g.slots) if 2 <= s_ <= 32]
         return [(s_, w_, self.chk) for s_ in slots for w_ in (2, 4, 8)]
 
-    def _refine(self, replays: int, tol: float = 0.02, passes: int = 2) -> None:
+    def _refine(self, replays: int, tol: float = 0.02, passes: int = 3) -> None:
         """Coordinate search over the groups' shapes on the captured graph: the
         tuner probes each group alone, where wide speculation fills the device,
         while in the graph the groups share it.  For each group in capture
@@ -204,20 +204,65 @@ class Batch:
         method 0: 4 or 8 waves, launches of a quarter, half or all of the
         sweeps) is timed over `replays` replays from the start states and kept
         if it beats the best so far by `tol` ms; up to `passes` passes, until
-        one changes nothing (the groups' choices interact: a second pass
-        moved the L = 256 group back to eight waves).  Each timing uses a
-        checked capture (`_capture_checked`).  The searches advance the RNG
-        streams and the lattices, so every group ends at attempt 0 of its
-        streams and at its start state.  At
-        the batched configuration the replay went from 5.16 to 4.92 ms in one
-        pass and to 4.76-4.78 in two (L = 32 on 8 slots, L = 48 on quarter
-        words with 16)."""
+        one changes nothing, every other pass visiting the groups in reverse
+        order.  The groups' choices interact, so the search runs from two
+        starts and keeps the faster end: the tuner's shapes, and the same with
+        every method-2 group's sweeps in one launch and every multi-wave
+        method-0 group on its widest candidate with the shortest launches.
+        At the batched configuration the search from the tuner's shapes
+        (L = 32 on 16 slots in two launches, L = 256 on 8 waves in one) ended,
+        in three processes, twice at 4.57-4.58 ms (L = 32 on 8 slots, L = 256
+        in launches of 5 sweeps) and once at 4.86 ms (L = 256 on 4 waves in
+        launches of 10; neither move paid without the other).  Each timing
+        uses a checked capture (`_capture_checked`).  The searches advance
+        the RNG streams and the lattices, so every group ends at attempt 0 of
+        its streams and at its start state."""
+        shape = lambda g: (g.slots, g.waves, g.launch_sweeps)
+        tuned = [shape(g) for g in self.groups]
         self.graph.close()
         best = self._capture_checked(replays)
         self.tuned_ms = best
-        for _ in range(passes):
+        best = self._search(replays, tol, passes, best, 1)
+        ends = [(best, [shape(g) for g in self.groups])]
+        second = []
+        for g, sh in zip(self.groups, tuned):
+            if g.method == 2:
+                second.append((sh[0], sh[1], self.chk))
+            elif g.method == 0 and g.slots > 64:
+                second.append(max(self._candidates(g), key=lambda c: (c[0], -c[2])))
+            else:
+                second.append(sh)
+        if second not in (tuned, ends[0][1]):
+            self.graph.close()
+            for k, g in enumerate(self.groups):
+                if shape(g) != second[k]:
+                    g.slots, g.waves, g.launch_sweeps = second[k]
+                    self._rebuild(k)
+            t = self._capture_checked(replays)
+            self.refine_log.append({"L": None, "from": "second start", "to": [list(x) for x in second], "ms": t})
+            t = self._search(replays, tol, passes, t, 2)
+            ends.append((t, [shape(g) for g in self.groups]))
+        best, chosen = min(ends, key=lambda e: e[0])
+        # fresh engines at the chosen shapes, an instance of the graph that
+        # replays at the searched speed, then every group back to attempt 0
+        # and its start state
+        self.graph.close()
+        for k, g in enumerate(self.groups):
+            g.slots, g.waves, g.launch_sweeps = chosen[k]
+            g.engine.close()
+            g.rng.free()
+            self._make_engine(k, fixed=True)
+        self.refined_ms = self._capture_checked(replays, ref=best)
+        for g in self.groups:
+            g.engine.set_attempt(0)
+            self._load_start(g)
+
+    def _search(self, replays: int, tol: float, passes: int, best: float, start: int) -> float:
+        """The coordinate passes of `_refine` from the groups' current shapes
+        (the graph captured at them); returns the best ms per replay."""
+        for p in range(passes):
             changed = False
-            for k in list(self.order):
+            for k in (list(self.order) if p % 2 == 0 else list(reversed(self.order))):
                 g = self.groups[k]
                 for cand in self._candidates(g):
                     old = (g.slots, g.waves, g.launch_sweeps)
@@ -231,7 +276,8 @@ class Batch:
                     except RuntimeError:
                         t = None
                     if t is not None and t < best - tol:
-                        self.refine_log.append({"L": g.L, "from": list(old), "to": list(cand), "ms": t})
+                        self.refine_log.append({"L": g.L, "from": list(old), "to": list(cand), "ms": t,
+                                                "start": start})
                         best, changed = t, True
                     else:
                         g.slots, g.waves, g.launch_sweeps = old
@@ -239,18 +285,7 @@ class Batch:
                         self._capture()
             if not changed:
                 break
-        # fresh engines at the chosen shapes, an instance of the graph that
-        # replays at the searched speed, then every group back to attempt 0
-        # and its start state
-        self.graph.close()
-        for k, g in enumerate(self.groups):
-            g.engine.close()
-            g.rng.free()
-            self._make_engine(k, fixed=True)
-        self.refined_ms = self._capture_checked(replays, ref=best)
-        for g in self.groups:
-            g.engine.set_attempt(0)
-            self._load_start(g)
+        return best
 
     def _capture_checked(self, replays: int, ref: float = None, tries: int = 4) -> float:
         """Captures the graph and times it from the start states.  Instances of
