@@ -612,6 +612,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
     const int L = a.L, N = a.N;
     const int slot = lane & (S - 1);
     const int gbase = lane & ~(S - 1);
+    // the lanes of this lane's group below it (mark-plane filter: does an
+    // earlier lane of the group hold an accepted attempt)
+    const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << gbase) - 1ull);
 
     int rl, r0;
     if constexpr (MULTI) {
@@ -790,9 +793,7 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                     // mark at its own site through the old mark its OR
                     // returned: of two ORs at one site in one instruction only
                     // the second sees the first, and that may be either lane.
-                    uint64_t rd = group_after<S>(m);
-                    asm volatile("" : "+s"(rd));
-                    const bool reads = __builtin_amdgcn_inverse_ballot_w64(rd);
+                    const bool reads = (m & below) != 0ull;
                     bool hA, hB;
                     if constexpr (LAT == LAT_MULTIHALF) {
                         // whole dwords: one test of the replica's bit at the end
