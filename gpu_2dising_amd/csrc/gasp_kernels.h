@@ -869,8 +869,25 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
             // Invalid attempts, as wave masks: every lane after the first
             // conflicting lane of its group (both attempts), and at a lane
             // its own conflicts (an invalid A also invalidates its B).
-            const uint64_t invA = group_after<S>(cA | cB) | cA;
-            const uint64_t invB = invA | cB;
+            // (a round without a conflict, the common case, skips the masks
+            // and the prefix lengths: nothing is invalid, all 2 S are valid)
+            const uint64_t cc = cA | cB;
+            uint64_t invA = 0ull, invB = 0ull;
+            uint32_t P = 2u * (uint32_t)S;
+            if (cc != 0ull) {
+                invA = group_after<S>(cc) | cA;
+                invB = invA | cB;
+                // The valid attempts of a group are a prefix of its order A,
+                // B, A, B, ... (an attempt is invalid once an earlier one
+                // conflicts); the next round starts after that prefix.
+                if constexpr (S == 64) {
+                    P = (uint32_t)(__popcll(~invA) + __popcll(~invB));
+                } else {
+                    constexpr uint32_t kOnes = (S == 32) ? 0xffffffffu : ((1u << S) - 1u);
+                    P = (uint32_t)__popc((uint32_t)(~invA >> gbase) & kOnes) +
+                        (uint32_t)__popc((uint32_t)(~invB >> gbase) & kOnes);
+                }
+            }
             if (__builtin_amdgcn_inverse_ballot_w64(mA & ~invA)) lat.flip(A.st.c, A.s01);
             if (__builtin_amdgcn_inverse_ballot_w64(mB & ~invB)) lat.flip(B.st.c, B.s01);
             if constexpr (MOFF > 0) {
@@ -879,19 +896,9 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                 if (__builtin_amdgcn_inverse_ballot_w64(mA)) unmark_site<LAT, MOFF>(A.st.c, rl);
                 if (__builtin_amdgcn_inverse_ballot_w64(mB)) unmark_site<LAT, MOFF>(B.st.c, rl);
             }
-            // The valid attempts of a group are a prefix of its order A, B,
-            // A, B, ... (an attempt is invalid once an earlier one conflicts);
-            // the next round starts after that prefix.  Disabled attempts
+            // The next round starts after the valid prefix.  Disabled attempts
             // before l0 are valid (nothing precedes them) and those at or past
             // e end past e, so the clamp to e covers them.
-            uint32_t P;
-            if constexpr (S == 64) {
-                P = (uint32_t)(__popcll(~invA) + __popcll(~invB));
-            } else {
-                constexpr uint32_t kOnes = (S == 32) ? 0xffffffffu : ((1u << S) - 1u);
-                P = (uint32_t)__popc((uint32_t)(~invA >> gbase) & kOnes) +
-                    (uint32_t)__popc((uint32_t)(~invB >> gbase) & kOnes);
-            }
             const uint32_t nxt = (l0 & ~1u) + P;
             l0 = nxt < e ? nxt : e;
             if constexpr (LAT == LAT_HBM) {
