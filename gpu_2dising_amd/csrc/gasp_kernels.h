@@ -788,7 +788,8 @@ __global__ void __launch_bounds__(1024) sweep_kernel(SweepArgs a) {
                     // Only the lanes after their group's first lane with an
                     // accepted attempt can follow an accepted attempt of
                     // another lane, so only they read their stencils' marks
-                    // (at config 2 about half the lanes skip the ten reads).
+                    // (at L = 64 with 16384 replicas on 16 slots about half
+                    // the lanes skip the ten reads).
                     // An accepted attempt among the others still reports a
                     // mark at its own site through the old mark its OR
                     // returned: of two ORs at one site in one instruction only
