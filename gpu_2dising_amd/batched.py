@@ -90,7 +90,7 @@ class Batch:
         if len(filters) != len(groups):
             raise ValueError("conflict_filter: one value, or one per group")
         self.staging, self.tables = lattice_staging, dependence_tables
-        self.refine_log, self.tuned_ms, self.refined_ms = [], None, None
+        self.refine_log, self.tuned_ms, self.search_ms, self.refined_ms = [], None, None, None
         set_lattice_staging(lattice_staging)
         set_dependence_tables(dependence_tables)
         try:
@@ -243,6 +243,7 @@ class Batch:
             t = self._search(replays, tol, passes, t, 2)
             ends.append((t, [shape(g) for g in self.groups]))
         best, chosen = min(ends, key=lambda e: e[0])
+        self.search_ms = best  # at most tuned_ms: each search only keeps faster shapes
         # fresh engines at the chosen shapes, an instance of the graph that
         # replays at the searched speed, then every group back to attempt 0
         # and its start state

@@ -186,9 +186,11 @@ def test_batched_configuration_full_shape(gpu, engine_lib, oracle):
 
 def test_batched_refined_graph(gpu, engine_lib, oracle):
     """Batch with the coordinate search on the captured graph: the chosen
-    shapes are valid and no slower than the tuner's (by the search's own
-    timings), and after the search every group starts again from attempt 0
-    of its streams: two replays, then for 6 replicas of every group the
+    shapes are valid and no slower than the tuner's by the search's own
+    timings (the graph rebuilt at them is timed again, over two replays here,
+    so that figure is only bounded loosely), and after the search every group
+    starts again from attempt 0 of its streams: two replays, then for 6
+    replicas of every group the
     oracle replays the same Philox streams from the start state."""
     from gpu_2dising_amd.batched import Batch
 
@@ -196,7 +198,8 @@ def test_batched_refined_graph(gpu, engine_lib, oracle):
     print("[batched] refined on the graph", flush=True)
     b = Batch(chk=20, seed=2026, refine=True, refine_replays=2)
     try:
-        assert b.tuned_ms is not None and b.refined_ms <= b.tuned_ms
+        assert b.tuned_ms is not None and b.search_ms <= b.tuned_ms
+        assert b.refined_ms < 1.5 * b.tuned_ms
         for g in b.groups:
             assert g.engine.shape() == (g.slots, g.waves), (g.L, g.engine.shape(), g.slots, g.waves)
         b.graph.launch(2)
