@@ -190,8 +190,8 @@ def test_batched_refined_graph(gpu, engine_lib, oracle):
     timings (the graph rebuilt at them is timed again, over two replays here,
     so that figure is only bounded loosely), and after the search every group
     starts again from attempt 0 of its streams: two replays, then for 6
-    replicas of every group the
-    oracle replays the same Philox streams from the start state."""
+    replicas of every group the oracle replays the same Philox streams from
+    the start state."""
     from gpu_2dising_amd.batched import Batch
 
     G = engine_lib
